@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Benchmark of the interleaved block-cyclic FFT + stencil path on one MI355X.
+
+Workload (default): 2D N=8192, P=4 (16 virtual ranks), B=32.  One step is the
+round trip of the reference's example() (zmodel/main.cu:50-80): 5-point
+Laplacian (h = 1/N) -> forward 2D FFT -> inverse 2D FFT scaled 1/N^2, on a
+seeded smooth field (256 random Fourier modes with |k| <= N/8, amplitudes
+N(0,1)/(1+|k|^2), uniform phases, plus 1e-3 uniform noise) held in HBM.
+
+Prints one JSON line on stdout (diagnostics go to stderr):
+  value       FFT GFLOP/s, 5 N^2 log2(N^2) per transform, two transforms per step
+  roofline    the step's dominant kernel, from HIP events recorded around each
+              launch on its stream during the timed region
+  cpu_baseline the FP64 CPU oracle (a C port of the reference path) on the
+              host cores, one full step of the same field
+  errors      relative L2 / max error of every FP32 output against the FP64
+              GPU path on the same FP32-rounded input
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def mode_field_gpu(n, nmodes, kmax, noise, seed, torch):
+    """Natural-layout complex128 field: exact sum of `nmodes` plane waves
+    (evaluated as a rank-`nmodes` complex GEMM in FP64) plus uniform noise."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ks, coef = [], []
+    while len(ks) < nmodes:
+        k = rng.integers(-kmax, kmax + 1, size=2)
+        k2 = float(k[0] ** 2 + k[1] ** 2)
+        if k2 > kmax * kmax:
+            continue
+        ks.append(k)
+        coef.append(rng.normal() / (1.0 + k2) * np.exp(1j * rng.uniform(0, 2 * np.pi)))
+    ks = np.array(ks)
+    pos = np.arange(n)
+    # exact integer phase reduction: (k * x) mod n
+    uy = np.exp(2j * np.pi * ((ks[:, 0][None, :] * pos[:, None]) % n) / n)  # (n, m): rows gr
+    vx = np.exp(2j * np.pi * ((ks[:, 1][None, :] * pos[:, None]) % n) / n)  # (n, m): cols gc
+    U = torch.from_numpy(uy * np.array(coef)[None, :]).cuda()
+    V = torch.from_numpy(np.ascontiguousarray(vx.T)).cuda()
+    f = U @ V
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    re = torch.rand((n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    im = torch.rand((n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    f += noise * torch.complex(re, im)
+    return f.reshape(-1).contiguous()
+
+
+def rel_errors(a, ref, torch):
+    d = (a.to(torch.complex128) - ref)
+    l2 = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
+    mx = float(d.abs().max() / ref.abs().max())
+    return {"rel_l2": l2, "rel_max": mx}
+
+
+def run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream):
+    for _ in range(steps):
+        plan.laplace(x, scale=float(n * n), out=lap, stream=stream)
+        plan.forward(lap, out=y, scratch=scratch, stream=stream)
+        plan.inverse(y, 1.0 / (n * n), out=z, scratch=scratch, stream=stream)
+
+
+def time_path(plan, x, n, steps, warmup, torch, label):
+    lap = torch.empty_like(x)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    scratch = torch.empty(plan.scratch_bytes // x.element_size(), dtype=x.dtype, device="cuda")
+    stream = torch.cuda.current_stream()
+    run_steps(plan, x, lap, y, z, scratch, n, warmup, torch, stream)
+    torch.cuda.synchronize()
+    plan.timing_reset()
+    plan.timing(True)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    plan.timing(False)
+    ms = start.elapsed_time(stop) / steps
+    stages = plan.timing_read()
+    plan.timing_reset()
+    log(f"{label}: {ms:.3f} ms/step (wall {1e3 * wall / steps:.3f} ms/step); stages " +
+        ", ".join(f"{k}: {c} x {t / c:.3f} ms" for k, (c, t) in stages.items()))
+    return ms, stages, (lap, y, z)
+
+
+def cpu_baseline(n, p, b, seed):
+    """FP64 CPU oracle (C port of the reference path, radix-2 DFT mode, OpenMP
+    over the host cores) on one full step of the same workload."""
+    import numpy as np
+
+    import oracle
+    oracle.build()
+    threads = len(os.sched_getaffinity(0))
+    s = oracle.Shape(2, n, p, b)
+    rng = np.random.default_rng(seed)
+    x = (rng.uniform(-1, 1, s.total) + 1j * rng.uniform(-1, 1, s.total)).astype(np.complex128)
+    oracle.set_dft_mode(1)
+    try:
+        t0 = time.perf_counter()
+        lap = s.laplace(x)
+        t1 = time.perf_counter()
+        y = s.forward_fft(lap)
+        t2 = time.perf_counter()
+        s.inverse_fft(y, scale=1.0 / (n * n))
+        t3 = time.perf_counter()
+    finally:
+        oracle.set_dft_mode(0)
+    flops = 2 * 5.0 * n * n * math.log2(n * n)
+    log(f"cpu oracle ({threads} threads): laplace {t1 - t0:.2f} s, forward {t2 - t1:.2f} s, inverse {t3 - t2:.2f} s")
+    return {
+        "value": flops / (t3 - t0) / 1e9,
+        "unit": "GFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one full step (5-point Laplacian, forward FFT, inverse FFT) of a 2D N={n} P={p} B={b} "
+                   f"complex128 field in the FP64 C oracle (radix-2 DFT mode, OpenMP); "
+                   f"{t3 - t0:.2f} s, forward alone {t2 - t1:.2f} s"),
+        "seconds_per_step": t3 - t0,
+        "forward_seconds": t2 - t1,
+    }
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_mfma"])
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--b", type=int, default=32)
+    ap.add_argument("--two-stage", action="store_true", help="force the two-kernel-per-direction path")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=8192)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs the whole P x P virtual rank grid on one GPU; "
+                                   "--gpus must be 1 (multi-GPU sharding is a design note, not built)"}))
+        sys.exit(2)
+
+    import torch
+
+    from stencilfft_amd import Plan
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU"}))
+        sys.exit(1)
+
+    n, p, b = args.n, args.p, args.b
+    log(f"workload 2D N={n} P={p} B={b} precision={args.precision} steps={args.steps} warmup={args.warmup}")
+    nat = mode_field_gpu(n, 256, n // 8, 1e-3, args.seed, torch)
+    plan64 = Plan(2, n, p, b, precision="fp64", two_stage=args.two_stage)
+    x64 = plan64.natural_to_packed(nat)
+    del nat
+    if args.precision == "fp64":
+        plan = plan64
+        x = x64
+    else:
+        plan = Plan(2, n, p, b, precision=args.precision, two_stage=args.two_stage)
+        x = x64.to(torch.complex64)
+    torch.cuda.synchronize()
+
+    ms, stages, outs = time_path(plan, x, n, args.steps, args.warmup, torch, args.precision)
+    esize = x.element_size()
+    field = n * n
+    flops = 2 * 5.0 * field * math.log2(field)
+    gflops = flops / (ms * 1e-3) / 1e9
+    # bytes per step: Section 8(d) accounting (2 sweeps per direction, fused
+    # pack/exchange/twiddle/repack) and the textbook bound (1 sweep per direction)
+    bytes_s8d = (2 * (2 * 2 * 2 * field * esize)) + 2 * field * esize
+    bytes_textbook = (2 * (2 * 2 * field * esize)) + 2 * field * esize
+    step_frac = bytes_s8d / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+
+    # dominant kernel and its roofline (textbook bytes: one read + one write of
+    # the field per launch; every stage kernel here moves exactly that)
+    dom = max(stages.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_count, dom_ms) = dom
+    dom_avg = dom_ms / dom_count
+    per_launch = 2 * field * esize
+    achieved = per_launch / (dom_avg * 1e-3) / 1e9
+    s8d_per_launch = {"line": 2 * per_launch}.get(dom_name, per_launch)
+    traffic = load_traffic(f"traffic_{dom_name}_n{n}_p{p}_b{b}_{args.precision}.json")
+    roofline = {
+        "bound": "hbm",
+        "kernel": {"line": "line_kernel (fused FFT1a+exchange+twiddle+FFT1b pass)",
+                   "repack": "repack_kernel", "laplace": "laplace_kernel",
+                   "stage_a": "stage_a_kernel", "stage_b": "stage_b_kernel"}.get(dom_name, dom_name),
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic["bytes_per_launch"] if traffic else None,
+        "bytes_per_launch": per_launch,
+        "avg_launch_ms": round(dom_avg, 4),
+        "launches": dom_count,
+        "basis": "textbook bytes: one read and one write of the N^2 field per launch",
+        "achieved_s8d": round(s8d_per_launch / (dom_avg * 1e-3) / 1e9, 1),
+        "bytes_per_launch_s8d": s8d_per_launch,
+    }
+
+    result_extra = {
+        "stage_ms": {k: round(t / c, 4) for k, (c, t) in stages.items()},
+        "stage_launches_per_step": {k: c // args.steps for k, (c, t) in stages.items()},
+        "step_bytes_s8d": bytes_s8d,
+        "step_bytes_textbook": bytes_textbook,
+        "step_hbm_frac_s8d": round(step_frac, 4),
+        "step_hbm_frac_textbook": round(bytes_textbook / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+    if not args.no_fp64 and args.precision != "fp64":
+        x64r = x.to(torch.complex128)
+        ms64, st64, outs64 = time_path(plan64, x64r, n, max(3, min(args.steps, 10)), 2, torch, "fp64")
+        errs = {}
+        for name, a, r in zip(("laplace", "forward", "round_trip"), outs, outs64):
+            errs[name] = rel_errors(a, r, torch)
+        # IFFT(FFT(lap)) against the FP64 Laplacian it started from
+        errs["round_trip_vs_fp64_laplace"] = rel_errors(outs[2], outs64[0], torch)
+        result_extra["errors_vs_fp64_gpu"] = errs
+        result_extra["fp64_ms_per_step"] = round(ms64, 4)
+        result_extra["speedup_vs_fp64_gpu"] = round(ms64 / ms, 3)
+        log(f"errors vs FP64 GPU path: {json.dumps(errs)}")
+        del x64r, outs64
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        del outs
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(n, p, b, args.seed)
+
+    line = {
+        "metric": "FFT GFLOP/s (5N·log2N), % HBM BW peak; FP32 vs FP64 path at stated error",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"2D N={n} P={p} B={b} ({p * p} virtual ranks): 5-point Laplacian (h=1/N) -> forward FFT "
+                         f"-> inverse FFT (1/N^2), complex {'128' if args.precision == 'fp64' else '64'}"),
+            "n": n, "p": p, "b": b, "precision": args.precision,
+            "path": "two-stage" if args.two_stage else "fused",
+            "field": f"256 modes |k|<=N/8, amp N(0,1)/(1+|k|^2), + 1e-3 U[-1,1) noise, seed {args.seed}",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **result_extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,424 @@
+// line_fast.hip -- the fused per-direction pass specialised at compile time for
+// the (VEC, M) pairs of the hot configurations.
+//
+// Same mathematics and layouts as line_kernel in fft.hip (one N-point DFT per
+// line, N = VEC * M, loaded from the packed real layout and stored in the
+// reciprocal layout of zmodel/transforms/fft.cu:162-181 +
+// fft_device_plans.h:64-66), organised for gfx950:
+//   * every extent is a compile-time power of two, so all index arithmetic is
+//     shifts and masks and LDS addresses fold into immediate offsets;
+//   * a workgroup loops over lines; while it transforms line i in LDS the
+//     global loads of line i+1 are already in flight into registers;
+//   * twiddles come from LDS tables (W_S for the sub-transforms, and a
+//     two-level W_N = A[e & 63] * B[e >> 6] for the inter-stage twiddle), all
+//     generated in FP64 on the GPU;
+//   * the last radix pass writes straight to HBM with lanes running along the
+//     contiguous output index.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace zm {
+
+namespace fast {
+
+template <int N>
+struct Log2 {
+    static constexpr int value = 1 + Log2<N / 2>::value;
+};
+template <>
+struct Log2<1> {
+    static constexpr int value = 0;
+};
+
+// radix of pass `p` when a length-2^LN transform is split greedily into
+// radix-16 passes followed by one smaller pass
+template <int LN, int P>
+struct Radix {
+    static constexpr int done = 4 * P;
+    static constexpr int left = LN - done;
+    static constexpr int value = left >= 4 ? 16 : (1 << (left > 0 ? left : 0));
+};
+template <int LN>
+struct NumPasses {
+    static constexpr int value = (LN + 3) / 4;
+};
+
+}  // namespace fast
+
+// Twiddle tables for a line of length N = VEC*M (S = max(VEC, M)):
+//   [0, S)          W_S^t
+//   [S, S+64)       W_N^t,      t < 64
+//   [S+64, S+64+N/64) W_N^(64 t)
+template <typename T>
+struct TwTables {
+    const Cx<T>* ts;
+    const Cx<T>* ta;
+    const Cx<T>* tb;
+};
+
+template <bool FWD, typename C>
+__device__ __forceinline__ C tw_n(const C* ta, const C* tb, int e) {
+    // W_N^e for 0 <= e < N from the two-level table
+    const C a = ta[e & 63], b = tb[e >> 6];
+    return ctw<FWD>(C{1, 0}, cmul(a, b));
+}
+
+// One Stockham radix-R pass over the line-group held in LDS.
+//   CONTIG: transforms are contiguous runs of length n (the VEC-point DFTs,
+//           batch index bt = (line, r)); lanes run along j.
+//   !CONTIG: element i of transform (line, k1) sits at line*N + k1 + i*VEC
+//           (the M-point DFTs); lanes run along k1.
+// PRE: multiply element i of transform bt by W_N^(i * kk(bt)) first.
+// TOGLOBAL: instead of writing LDS, hand each output to `sink`.
+template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
+          bool TOGLOBAL, typename Sink>
+__device__ __forceinline__ void fast_pass(Cx<T>* lds, const Cx<T>* ts, int ls, const Cx<T>* ta, const Cx<T>* tb,
+                                          int tid, Sink&& sink) {
+    using C = Cx<T>;
+    constexpr int n = 1 << LN_T;
+    constexpr int nbf = n / R;
+    constexpr int LNBF = fast::Log2<nbf>::value;
+    constexpr int VEC = 1 << LVEC;
+    constexpr int N = 1 << LN;
+    constexpr int E = 1 << LE;
+    constexpr int BPT = EPT / R;
+    constexpr int ES = CONTIG ? 1 : VEC;
+    static_assert(BPT * NT * R == E, "work split");
+    C v[BPT][R];
+    int obase[BPT];
+#pragma unroll
+    for (int k = 0; k < BPT; k++) {
+        const int u = tid + k * NT;
+        int base, j, kk;
+        if (CONTIG) {
+            j = u & (nbf - 1);
+            const int bt = u >> LNBF;  // (line, r) when n == VEC
+            base = bt << LN_T;
+            kk = bt & ((N >> LN_T) - 1);  // r for the VEC transforms
+        } else {
+            const int k1 = u & (VEC - 1);
+            const int rest = u >> LVEC;
+            j = rest & (nbf - 1);
+            const int line = rest >> LNBF;
+            base = (line << LN) + k1;
+            kk = k1;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) v[k][r] = lds[pidx(base + (j + r * nbf) * ES)];
+        if (PRE) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int e = ((j + r * nbf) * kk) & (N - 1);
+                v[k][r] = ctw<FWD>(v[k][r], cmul(ta[e & 63], tb[e >> 6]));
+            }
+        }
+        if (NS > 1) {
+            const int jm = j & (NS - 1);
+            // W_{NS*R}^(jm*r) = W_S^(jm*r*S/(NS*R)), S = 2^ls
+            const int step = (1 << ls) / (NS * R);
+#pragma unroll
+            for (int r = 1; r < R; r++) v[k][r] = ctw<FWD>(v[k][r], ts[jm * r * step]);
+        }
+        dft<R, FWD>(v[k]);
+        const int jm = j & (NS - 1);
+        const int oidx = (j - jm) * R + jm;  // output index within the transform, before + r*NS
+        obase[k] = TOGLOBAL ? oidx : base + oidx * ES;
+    }
+    if (!TOGLOBAL) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < BPT; k++) {
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[pidx(obase[k] + r * NS * ES)] = v[k][r];
+        }
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int k = 0; k < BPT; k++) {
+            const int u = tid + k * NT;
+#pragma unroll
+            for (int r = 0; r < R; r++) sink(u, obase[k] + r * NS, v[k][r]);
+        }
+    }
+}
+
+struct NoSink {
+    template <typename C>
+    __device__ void operator()(int, int, C) const {}
+};
+
+// Run passes FIRST..LAST-1 of a length-2^LN_T transform (all in LDS).
+template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE>
+__device__ __forceinline__ void fast_passes_lds(Cx<T>* lds, const Cx<T>* ts, int ls, const Cx<T>* ta,
+                                                const Cx<T>* tb, int tid) {
+    if constexpr (PASS < NPASS) {
+        constexpr int R = fast::Radix<LN_T, PASS>::value;
+        constexpr int NS = 1 << (4 * PASS);
+        fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false>(lds, ts, ls, ta, tb, tid,
+                                                                                           NoSink{});
+        fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE>(lds, ts, ls, ta, tb, tid);
+    }
+}
+
+// Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
+// iteration.  a.lpb is the number of iterations (line groups) per workgroup.
+template <typename T, int LVEC, int LM, int LLPI, bool FWD>
+__global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
+    line_fast_kernel(const Cx<T>* in, Cx<T>* out, Geo g, LineArgs a, const Cx<T>* __restrict__ tables, T scale,
+                     int apply_scale, int groups) {
+    using C = Cx<T>;
+    constexpr int VEC = 1 << LVEC, M = 1 << LM;
+    constexpr int LN = LVEC + LM, N = 1 << LN;
+    constexpr int LE = LN + LLPI, E = 1 << LE;
+    constexpr int NT = E / EPT;
+    constexpr int LS = LVEC > LM ? LVEC : LM, S = 1 << LS;
+    constexpr int NPV = fast::NumPasses<LVEC>::value, NPM = fast::NumPasses<LM>::value;
+    constexpr int NTAB = S + 64 + (N / 64 > 0 ? N / 64 : 1);
+    extern __shared__ unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    C* tabs = lds + padded_elems(E);
+    const int tid = threadIdx.x;
+
+    for (int t = tid; t < NTAB; t += NT) tabs[t] = tables[t];
+    const C* ts = tabs;
+    const C* ta = tabs + S;
+    const C* tb = tabs + S + 64;
+
+    const int L = g.l, ll = g.ll, lp = g.lp;
+    const int lvp = g.lvec - g.lp;
+    const int VP = 1 << lvp;
+    int lsg = 0;
+    while ((1 << lsg) < a.sg) lsg++;
+    const int lpg = a.lines_per_group;
+    const C* src0 = in + (int64_t)blockIdx.y * g.field;
+    C* dst0 = out + (int64_t)blockIdx.y * g.field;
+
+    // global offset (in elements, within the field) of element `pos` of member
+    // `gm`'s row q of group o
+    auto row_addr = [&](int o, int gm, int q, int pos) -> int64_t {
+        const int rank = ((o >> lsg) << (lsg + lp)) + (o & (a.sg - 1)) + (gm << lsg);
+        return (int64_t)rank * g.local + ((int64_t)q << ll) + pos;
+    };
+
+    // register prefetch of one line group: element e = tid + k*NT
+    C pre[EPT];
+    auto fetch = [&](int grp) {
+        const int64_t lam0 = (int64_t)grp << LLPI;
+        const int o = (int)(lam0 / lpg);
+        const int q0 = (int)(lam0 - (int64_t)o * lpg);
+#pragma unroll
+        for (int k = 0; k < EPT; k++) {
+            const int e = tid + k * NT;
+            const int l = e >> LN, rem = e & (N - 1);
+            const int gm = rem >> ll, pos = rem & (L - 1);
+            pre[k] = src0[row_addr(o, gm, q0 + l, pos)];
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int k = 0; k < EPT; k++) {
+            const int e = tid + k * NT;
+            int li = e;
+            if (!FWD) {
+                const int l = e >> LN, rem = e & (N - 1);
+                const int gm = rem >> ll, pos = rem & (L - 1);
+                li = (l << LN) + ((pos >> lvp) << LVEC) + (gm << lvp) + (pos & (VP - 1));
+            }
+            lds[pidx(li)] = pre[k];
+        }
+    };
+
+    int grp = blockIdx.x;
+    if (grp < groups) fetch(grp);
+    __syncthreads();  // tables
+    for (; grp < groups; grp += gridDim.x) {
+        stage();
+        __syncthreads();
+        const int nxt = grp + gridDim.x;
+        if (nxt < groups) fetch(nxt);
+
+        const int64_t lam0 = (int64_t)grp << LLPI;
+        const int o = (int)(lam0 / lpg);
+        const int q0 = (int)(lam0 - (int64_t)o * lpg);
+
+        if constexpr (FWD) {
+            // VEC-point DFTs (contiguous), all passes in LDS
+            fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false>(lds, ts, LS, ta, tb, tid);
+            // M-point DFTs (stride VEC) with the inter-stage twiddle on the
+            // first pass; the last pass stores to HBM
+            fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true>(lds, ts, LS, ta, tb, tid);
+            constexpr int PL = NPM - 1;
+            constexpr int RL = fast::Radix<LM, PL>::value;
+            constexpr int NSL = 1 << (4 * PL);
+            fast_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, true>(
+                lds, ts, LS, ta, tb, tid, [&](int u, int k2, C v) {
+                    // transform (line, k1): k1 = u & (VEC-1), line = u >> (LVEC + log2(nbf))
+                    constexpr int nbf = (1 << LM) / RL;
+                    const int k1 = u & (VEC - 1);
+                    const int line = (u >> LVEC) >> fast::Log2<nbf>::value;
+                    const int gq = k1 >> lvp, fr = k1 & (VP - 1);
+                    dst0[row_addr(o, gq, q0 + line, fr + (k2 << lvp))] = v;
+                });
+        } else {
+            // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
+            // the conjugate inter-stage twiddle; the last pass stores to HBM
+            fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false>(lds, ts, LS, ta, tb, tid);
+            fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true>(lds, ts, LS, ta, tb, tid);
+            constexpr int PL = NPV - 1;
+            constexpr int RL = fast::Radix<LVEC, PL>::value;
+            constexpr int NSL = 1 << (4 * PL);
+            const T sc = scale;
+            fast_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, true>(
+                lds, ts, LS, ta, tb, tid, [&](int u, int jj, C v) {
+                    // transform (line, r): r = bt & (M-1), line = bt >> LM
+                    constexpr int nbf = (1 << LVEC) / RL;
+                    const int bt = u >> fast::Log2<nbf>::value;
+                    const int r = bt & (M - 1), line = bt >> LM;
+                    const int gm = r >> g.lb, j = r & (g.b - 1);
+                    if (apply_scale) v = C{v.x * sc, v.y * sc};
+                    dst0[row_addr(o, gm, q0 + line, (j << LVEC) + jj)] = v;
+                });
+        }
+        __syncthreads();  // the next stage() overwrites LDS
+    }
+}
+
+// host side ------------------------------------------------------------------
+
+template <typename T, int LVEC, int LM, int LLPI, bool FWD>
+static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a,
+                                   const Cx<T>* tables, int batch, double scale, int grid_x, hipStream_t stream) {
+    constexpr int LN = LVEC + LM, N = 1 << LN, E = N << LLPI;
+    constexpr int S = (1 << (LVEC > LM ? LVEC : LM));
+    constexpr int NTAB = S + 64 + (N / 64 > 0 ? N / 64 : 1);
+    const size_t lds = sizeof(Cx<T>) * (padded_elems(E) + NTAB);
+    const void* fn = (const void*)&line_fast_kernel<T, LVEC, LM, LLPI, FWD>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int groups = (int)(g.field / N) >> LLPI;
+    const dim3 grid((unsigned)(grid_x < groups ? grid_x : groups), (unsigned)batch);
+    hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD>), grid, dim3(E / EPT), lds, stream, in, out, g, a,
+                       tables, (T)scale, scale != 1.0 ? 1 : 0, groups);
+    return hipGetLastError();
+}
+
+template <typename T, int LVEC, int LM>
+static hipError_t launch_fast_lm(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
+                                 int batch, bool fwd, double scale, int grid_x, hipStream_t stream) {
+    // lines per iteration: E = N * LPI ~ 4096 elements, at least one line
+    constexpr int LN = LVEC + LM;
+    constexpr int LLPI = LN >= 12 ? 0 : 12 - LN;
+    if (fwd) return launch_fast_inst<T, LVEC, LM, LLPI, true>(in, out, g, a, tables, batch, scale, grid_x, stream);
+    return launch_fast_inst<T, LVEC, LM, LLPI, false>(in, out, g, a, tables, batch, scale, grid_x, stream);
+}
+
+template <typename T>
+bool fast_line_supported(int vec, int m) {
+    const bool okv = vec == 16 || vec == 32 || vec == 64 || vec == 128;
+    const bool okm = m == 16 || m == 32 || m == 64 || m == 128;
+    return okv && okm && (int64_t)vec * m <= 8192;
+}
+
+template <typename T>
+size_t fast_line_lds_bytes(int vec, int m) {
+    const int n = vec * m;
+    int llpi = 0;
+    while ((n << llpi) < 4096) llpi++;
+    const int s = vec > m ? vec : m;
+    return sizeof(Cx<T>) * (padded_elems(n << llpi) + s + 64 + (n / 64 > 0 ? n / 64 : 1));
+}
+
+template <typename T>
+int fast_line_lines_per_iter(int vec, int m) {
+    int llpi = 0;
+    while ((vec * m << llpi) < 4096) llpi++;
+    return 1 << llpi;
+}
+
+template <typename T>
+hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
+                            int batch, bool fwd, double scale, int grid_x, hipStream_t stream) {
+#define ZM_LF(V, MM)                                                                                    \
+    if (g.vec == (1 << V) && g.m == (1 << MM))                                                          \
+        return launch_fast_lm<T, V, MM>(in, out, g, a, tables, batch, fwd, scale, grid_x, stream);
+    ZM_LF(4, 4) ZM_LF(4, 5) ZM_LF(4, 6) ZM_LF(4, 7)
+    ZM_LF(5, 4) ZM_LF(5, 5) ZM_LF(5, 6) ZM_LF(5, 7)
+    ZM_LF(6, 4) ZM_LF(6, 5) ZM_LF(6, 6) ZM_LF(6, 7)
+    ZM_LF(7, 4) ZM_LF(7, 5) ZM_LF(7, 6)
+#undef ZM_LF
+    return hipErrorInvalidValue;
+}
+
+// FP64 twiddle tables of a line (length N = vec*m), rounded to T: see TwTables.
+__global__ void line_tables_kernel(double2* t64, float2* t32, int n, int s) {
+    const int nt = s + 64 + (n / 64 > 0 ? n / 64 : 1);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double num, den;
+    if (t < s) {
+        num = t;
+        den = s;
+    } else if (t < s + 64) {
+        num = t - s;
+        den = n;
+    } else {
+        num = 64.0 * (t - s - 64);
+        den = n;
+    }
+    double sn, cs;
+    sincospi(-2.0 * num / den, &sn, &cs);
+    t64[t] = double2{cs, sn};
+    t32[t] = float2{(float)cs, (float)sn};
+}
+
+int line_table_entries(int vec, int m) {
+    const int n = vec * m, s = vec > m ? vec : m;
+    return s + 64 + (n / 64 > 0 ? n / 64 : 1);
+}
+
+hipError_t launch_line_tables(double2* t64, float2* t32, int vec, int m, hipStream_t stream) {
+    const int nt = line_table_entries(vec, m);
+    hipLaunchKernelGGL(line_tables_kernel, dim3((nt + 255) / 256), dim3(256), 0, stream, t64, t32, vec * m,
+                       vec > m ? vec : m);
+    return hipGetLastError();
+}
+
+template <typename T>
+int fast_line_occupancy(int vec, int m, bool fwd) {
+    int blocks = 0;
+    const size_t lds = fast_line_lds_bytes<T>(vec, m);
+    const int nthreads = vec * m * fast_line_lines_per_iter<T>(vec, m) / EPT;
+    const void* fn = nullptr;
+#define ZM_LF(V, MM)                                                                                       \
+    if (vec == (1 << V) && m == (1 << MM)) {                                                               \
+        constexpr int LL = (V + MM) >= 12 ? 0 : 12 - (V + MM);                                             \
+        fn = fwd ? (const void*)&line_fast_kernel<T, V, MM, LL, true>                                      \
+                 : (const void*)&line_fast_kernel<T, V, MM, LL, false>;                                    \
+    }
+    ZM_LF(4, 4) ZM_LF(4, 5) ZM_LF(4, 6) ZM_LF(4, 7)
+    ZM_LF(5, 4) ZM_LF(5, 5) ZM_LF(5, 6) ZM_LF(5, 7)
+    ZM_LF(6, 4) ZM_LF(6, 5) ZM_LF(6, 6) ZM_LF(6, 7)
+    ZM_LF(7, 4) ZM_LF(7, 5) ZM_LF(7, 6)
+#undef ZM_LF
+    if (!fn) return 0;
+    if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, nthreads, lds) != hipSuccess) return 0;
+    return blocks;
+}
+
+template bool fast_line_supported<float>(int, int);
+template bool fast_line_supported<double>(int, int);
+template size_t fast_line_lds_bytes<float>(int, int);
+template size_t fast_line_lds_bytes<double>(int, int);
+template int fast_line_lines_per_iter<float>(int, int);
+template int fast_line_lines_per_iter<double>(int, int);
+template int fast_line_occupancy<float>(int, int, bool);
+template int fast_line_occupancy<double>(int, int, bool);
+template hipError_t launch_line_fast<float>(const float2*, float2*, const Geo&, const LineArgs&, const float2*, int,
+                                            bool, double, int, hipStream_t);
+template hipError_t launch_line_fast<double>(const double2*, double2*, const Geo&, const LineArgs&, const double2*,
+                                             int, bool, double, int, hipStream_t);
+
+}  // namespace zm

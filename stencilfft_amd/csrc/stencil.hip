@@ -1,0 +1,225 @@
+// stencil.hip -- periodic stencils on the real-space packed layout of the
+// whole virtual grid: 5-point / 7-point Laplacian (zmodel/stencils/laplace.cu:
+// 73-169, laplace3d.cu:90-225) and the 4th-order x / y derivatives
+// (zmodel/stencils/dx.cu:10-82, dy.cu:10-103).
+//
+// The reference packs halos (pack_laplace, laplace.cu:172-191), exchanges them
+// with MPI_Sendrecv (laplace.cu:194-256) and reads them from receive buffers.
+// On one GPU the neighbour rank's slab is in the same buffer, so a halo read
+// is a direct load from that slab: the walks below return, for a step of d
+// samples along a global axis, the rank and local index holding the
+// neighbour, including the lane shift the reference applies at the global x
+// boundary (laplace.cu:148, 160; dx.cu:32, 57).
+//
+// Arithmetic follows the reference's evaluation order term by term with FMA
+// contraction off, so the FP64 results are bit-identical to it.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace zm {
+
+template <typename E>
+struct EOps;
+template <>
+struct EOps<float> {
+    using S = float;
+    static __device__ __forceinline__ float zero() { return 0.f; }
+    static __device__ __forceinline__ float add(float a, float b) { return a + b; }
+    static __device__ __forceinline__ float sub(float a, float b) { return a - b; }
+    static __device__ __forceinline__ float mul(float a, float s) { return a * s; }
+    static __device__ __forceinline__ float div(float a, float s) { return a / s; }
+};
+template <>
+struct EOps<double> {
+    using S = double;
+    static __device__ __forceinline__ double zero() { return 0.0; }
+    static __device__ __forceinline__ double add(double a, double b) { return a + b; }
+    static __device__ __forceinline__ double sub(double a, double b) { return a - b; }
+    static __device__ __forceinline__ double mul(double a, double s) { return a * s; }
+    static __device__ __forceinline__ double div(double a, double s) { return a / s; }
+};
+template <typename C, typename T>
+struct EOpsC {
+    using S = T;
+    static __device__ __forceinline__ C zero() { return C{(T)0, (T)0}; }
+    static __device__ __forceinline__ C add(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+    static __device__ __forceinline__ C sub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+    static __device__ __forceinline__ C mul(C a, T s) { return C{a.x * s, a.y * s}; }
+    static __device__ __forceinline__ C div(C a, T s) { return C{a.x / s, a.y / s}; }
+};
+template <>
+struct EOps<float2> : EOpsC<float2, float> {};
+template <>
+struct EOps<double2> : EOpsC<double2, double> {};
+
+// Neighbour along a block-cyclic axis (rows in 2D/3D, depth in 3D): local
+// index li of the rank at coordinate rc, step d (|d| <= B).  Inside a block it
+// stays on the rank; across a block edge it moves to rank rc -+ 1 and, at the
+// global edge (rc = 0 or P-1), to the previous / next block of that rank
+// (laplace.cu:103-140, dy.cu:34-97).
+__device__ __forceinline__ void bc_walk(const Geo& g, int rc, int li, int d, int* rc2, int* li2) {
+    const int B = g.b, P = g.p, nblk = g.l >> g.lb;
+    const int bk = li >> g.lb, t = (li & (B - 1)) + d;
+    if (t >= 0 && t < B) {
+        *rc2 = rc;
+        *li2 = (bk << g.lb) + t;
+    } else if (t < 0) {
+        *rc2 = (rc - 1 + P) & (P - 1);
+        const int bk2 = (rc == 0) ? ((bk - 1 + nblk) & (nblk - 1)) : bk;
+        *li2 = (bk2 << g.lb) + t + B;
+    } else {
+        *rc2 = (rc + 1) & (P - 1);
+        const int bk2 = (rc == P - 1) ? ((bk + 1) & (nblk - 1)) : bk;
+        *li2 = (bk2 << g.lb) + t - B;
+    }
+}
+
+// Neighbour along the packed x axis: vector j, lane jj of the rank at column
+// coordinate cid, step d (|d| <= B).  Across the rank edge it moves to the
+// neighbour rank's vector on the other side; at the global edge the lane
+// shifts by one (laplace.cu:142-165, dx.cu:29-77).
+__device__ __forceinline__ void x_walk(const Geo& g, int cid, int j, int jj, int d, int* cid2, int* lc2) {
+    const int B = g.b, P = g.p, VEC = g.vec;
+    const int t = j + d;
+    if (t >= 0 && t < B) {
+        *cid2 = cid;
+        *lc2 = (t << g.lvec) + jj;
+    } else if (t < 0) {
+        *cid2 = (cid - 1 + P) & (P - 1);
+        const int jj2 = (cid == 0) ? ((jj - 1 + VEC) & (VEC - 1)) : jj;
+        *lc2 = ((t + B) << g.lvec) + jj2;
+    } else {
+        *cid2 = (cid + 1) & (P - 1);
+        const int jj2 = (cid == P - 1) ? ((jj + 1) & (VEC - 1)) : jj;
+        *lc2 = ((t - B) << g.lvec) + jj2;
+    }
+}
+
+// coordinates of element i of a field
+struct Pt {
+    int did, rid, cid;
+    int ld, lr, lc;
+};
+
+__device__ __forceinline__ Pt decode(const Geo& g, int i) {
+    Pt p;
+    const int id = (int)(i / g.local);
+    const int rem = (int)(i - (int64_t)id * g.local);
+    p.lc = rem & (g.l - 1);
+    p.lr = (rem >> g.ll) & (g.l - 1);
+    p.ld = (g.dims == 3) ? (rem >> (2 * g.ll)) : 0;
+    p.cid = id & (g.p - 1);
+    p.rid = (id >> g.lp) & (g.p - 1);
+    p.did = (g.dims == 3) ? (id >> (2 * g.lp)) : 0;
+    return p;
+}
+
+__device__ __forceinline__ int64_t encode(const Geo& g, int did, int rid, int cid, int ld, int lr, int lc) {
+    const int id = (g.dims == 3) ? ((did << (2 * g.lp)) + (rid << g.lp) + cid) : ((rid << g.lp) + cid);
+    return (int64_t)id * g.local + (((int64_t)ld << g.ll) + lr) * g.l + lc;
+}
+
+template <typename E, typename S>
+__global__ void __launch_bounds__(256) laplace_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, S scale,
+                                                      int apply_scale) {
+#pragma clang fp contract(off)
+    using O = EOps<E>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.field) return;
+    const E* src = in + (int64_t)blockIdx.y * g.field;
+    const Pt p = decode(g, (int)i);
+    int r2, l2, c2, x2;
+    E s = O::zero();
+    // above / below: block-cyclic rows (laplace.cu:100-140)
+    bc_walk(g, p.rid, p.lr, -1, &r2, &l2);
+    s = O::add(s, src[encode(g, p.did, r2, p.cid, p.ld, l2, p.lc)]);
+    bc_walk(g, p.rid, p.lr, +1, &r2, &l2);
+    s = O::add(s, src[encode(g, p.did, r2, p.cid, p.ld, l2, p.lc)]);
+    // left / right: packed columns (laplace.cu:142-165)
+    const int j = p.lc >> g.lvec, jj = p.lc & (g.vec - 1);
+    x_walk(g, p.cid, j, jj, -1, &c2, &x2);
+    s = O::add(s, src[encode(g, p.did, p.rid, c2, p.ld, p.lr, x2)]);
+    x_walk(g, p.cid, j, jj, +1, &c2, &x2);
+    s = O::add(s, src[encode(g, p.did, p.rid, c2, p.ld, p.lr, x2)]);
+    const E c = src[i];
+    if (g.dims == 3) {
+        // front / back: block-cyclic depth (laplace3d.cu:181-219)
+        bc_walk(g, p.did, p.ld, -1, &r2, &l2);
+        s = O::add(s, src[encode(g, r2, p.rid, p.cid, l2, p.lr, p.lc)]);
+        bc_walk(g, p.did, p.ld, +1, &r2, &l2);
+        s = O::add(s, src[encode(g, r2, p.rid, p.cid, l2, p.lr, p.lc)]);
+        s = O::sub(s, O::mul(c, (S)6));
+    } else {
+        s = O::sub(s, O::mul(c, (S)4));
+    }
+    if (apply_scale) s = O::mul(s, scale);
+    out[(int64_t)blockIdx.y * g.field + i] = s;
+}
+
+// [1, -8, 0, 8, -1] / (12 delta) in the reference's order (dx.cu:29-79):
+// s = -8 f(-1) + f(-2) + 8 f(+1) - f(+2), then s / (12 delta).
+template <typename E, typename S, bool ALONG_X>
+__global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, S denom,
+                                                     int negate) {
+#pragma clang fp contract(off)
+    using O = EOps<E>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.field) return;
+    const E* src = in + (int64_t)blockIdx.y * g.field;
+    const Pt p = decode(g, (int)i);
+    E f[4];  // f(-1), f(-2), f(+1), f(+2)
+    const int steps[4] = {-1, -2, 1, 2};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int a, b2;
+        if (ALONG_X) {
+            x_walk(g, p.cid, p.lc >> g.lvec, p.lc & (g.vec - 1), steps[k], &a, &b2);
+            f[k] = src[encode(g, 0, p.rid, a, 0, p.lr, b2)];
+        } else {
+            bc_walk(g, p.rid, p.lr, steps[k], &a, &b2);
+            f[k] = src[encode(g, 0, a, p.cid, 0, b2, p.lc)];
+        }
+    }
+    E s = O::zero();
+    s = O::add(s, O::mul(f[0], (S)-8));
+    s = O::add(s, f[1]);
+    s = O::add(s, O::mul(f[2], (S)8));
+    s = O::sub(s, f[3]);
+    s = O::div(s, denom);
+    if (negate) s = O::mul(s, (S)-1);
+    out[(int64_t)blockIdx.y * g.field + i] = s;
+}
+
+template <typename E, typename S>
+static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int batch, StencilOp op, double coef,
+                                   bool negate, hipStream_t stream) {
+    const unsigned threads = 256;
+    const dim3 grid((unsigned)((g.field + threads - 1) / threads), (unsigned)batch);
+    const E* src = static_cast<const E*>(in);
+    E* dst = static_cast<E*>(out);
+    if (op == OP_LAPLACE) {
+        hipLaunchKernelGGL((laplace_kernel<E, S>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
+                           coef != 1.0 ? 1 : 0);
+    } else if (op == OP_DX4) {
+        hipLaunchKernelGGL((deriv4_kernel<E, S, true>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
+                           negate ? 1 : 0);
+    } else {
+        hipLaunchKernelGGL((deriv4_kernel<E, S, false>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
+                           negate ? 1 : 0);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_stencil(const void* in, void* out, const Geo& g, int batch, StencilOp op, bool complex_elems,
+                          double coef, bool negate, hipStream_t stream) {
+    if (complex_elems) return stencil_dispatch<Cx<T>, T>(in, out, g, batch, op, coef, negate, stream);
+    return stencil_dispatch<T, T>(in, out, g, batch, op, coef, negate, stream);
+}
+
+template hipError_t launch_stencil<float>(const void*, void*, const Geo&, int, StencilOp, bool, double, bool,
+                                          hipStream_t);
+template hipError_t launch_stencil<double>(const void*, void*, const Geo&, int, StencilOp, bool, double, bool,
+                                           hipStream_t);
+
+}  // namespace zm

@@ -1,0 +1,119 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the
+header declares, validates shapes with the documented status codes, and its
+host index maps agree with the oracle's restatement of zmodel/utils.h."""
+import ctypes
+import itertools
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, has_gpu
+
+HEADER = os.path.join(ROOT, "include", "stencilfft.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(zm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from stencilfft_amd import _lib
+    lib = _lib._lib()
+    syms = declared_symbols()
+    assert len(syms) >= 35
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # and the binding covers every declared symbol
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_version_and_strerror():
+    from stencilfft_amd import _lib, version
+    assert "gfx950" in version()
+    for st in range(8):
+        assert _lib._lib().zm_strerror(st)
+
+
+@pytest.mark.parametrize("dims,n,p,b,status", [
+    (2, 1024, 2, 16, 0), (2, 8192, 4, 32, 0), (3, 512, 2, 16, 0), (2, 16, 2, 2, 0),
+    (2, 100, 3, 2, 2),   # P does not divide N
+    (2, 64, 2, 3, 2),    # B does not divide L
+    (2, 64, 4, 8, 2),    # VEC = 2 not divisible by P = 4
+    (4, 64, 2, 2, 2),    # dims
+    (2, 0, 2, 2, 2),
+    (2, 96, 2, 2, 3),    # valid for the reference, not a power of two
+    (3, 2048, 2, 2, 3),  # N^3 beyond 32-bit element indices
+])
+def test_shape_check_status(dims, n, p, b, status):
+    from stencilfft_amd import Shape, ZmError
+    if status == 0:
+        d = Shape(dims, n, p, b).check()
+        assert d.l == n // p and d.vec == d.l // b and d.vp == d.vec // p and d.m == b * p
+        assert d.field == n ** dims
+    else:
+        with pytest.raises(ZmError) as e:
+            Shape(dims, n, p, b).check()
+        assert e.value.status == status
+
+
+@pytest.mark.parametrize("n,p,b", [(16, 2, 2), (32, 2, 4), (64, 4, 2), (64, 2, 1), (32, 2, 8)])
+def test_host_maps_match_oracle_2d(oracle_mod, n, p, b):
+    from stencilfft_amd import Shape
+    s = Shape(2, n, p, b)
+    o = oracle_mod.Shape(2, n, p, b)
+    for rid, cid in itertools.product(range(p), range(p)):
+        for lr, lc in itertools.product(range(s.check().l), repeat=2):
+            assert s.real_to_global(rid, cid, lr, lc) == o.real_to_global(rid, cid, lr, lc)
+            assert s.reciprocal_to_global(rid, cid, lr, lc) == o.reciprocal_to_global(rid, cid, lr, lc)
+    for g1, g2 in itertools.product(range(n), repeat=2):
+        assert s.global_to_real(g1, g2) == o.global_to_real(g1, g2)
+        assert s.global_to_reciprocal(g1, g2) == o.global_to_reciprocal(g1, g2)
+    for i in range(n):
+        assert s.ct(i) == o.ct(i) and s.ct_inv(i) == o.ct_inv(i)
+        assert s.packed_to_block_cyclic(i % s.check().l) == o.packed_to_block_cyclic(i % s.check().l)
+        assert s.block_cyclic_to_packed(i % s.check().l) == o.block_cyclic_to_packed(i % s.check().l)
+    for i in range(s.check().l):
+        assert s.lp(i) == o.lp(i) and s.lp_inv(i) == o.lp_inv(i)
+
+
+def test_host_maps_match_oracle_3d(oracle_mod):
+    from stencilfft_amd import Shape
+    n, p, b = 16, 2, 2
+    s = Shape(3, n, p, b)
+    o = oracle_mod.Shape(3, n, p, b)
+    for rid, cid, did in itertools.product(range(p), repeat=3):
+        for ld, lr, lc in itertools.product(range(8), repeat=3):
+            a = (rid, cid, did, ld, lr, lc)
+            assert s.real_to_global_3d(*a) == o.real_to_global_3d(*a)
+            assert s.reciprocal_to_global_3d(*a) == o.reciprocal_to_global_3d(*a)
+    for g in itertools.product(range(n), repeat=3):
+        assert s.global_to_real_3d(*g) == o.global_to_real_3d(*g)
+        assert s.global_to_reciprocal_3d(*g) == o.global_to_reciprocal_3d(*g)
+
+
+def test_plan_create_validates_before_touching_the_device():
+    from stencilfft_amd import _lib as L
+    h = ctypes.c_void_p()
+    bad = L.ZmShape(2, 100, 3, 2)
+    assert L._lib().zm_plan_create(ctypes.byref(h), ctypes.byref(bad), 0, 1, 0) == L.ZM_ERR_SHAPE
+    good = L.ZmShape(2, 64, 2, 2)
+    assert L._lib().zm_plan_create(ctypes.byref(h), ctypes.byref(good), 7, 1, 0) == L.ZM_ERR_INVALID_ARG
+    assert L._lib().zm_plan_create(ctypes.byref(h), ctypes.byref(good), 0, 0, 0) == L.ZM_ERR_INVALID_ARG
+    assert L._lib().zm_plan_create(None, ctypes.byref(good), 0, 1, 0) == L.ZM_ERR_INVALID_ARG
+    assert L._lib().zm_forward(None, None, None, None, None) == L.ZM_ERR_INVALID_ARG
+    assert L._lib().zm_laplace(None, None, None, 1.0, 0, None) == L.ZM_ERR_INVALID_ARG
+    if not has_gpu():
+        assert L._lib().zm_plan_create(ctypes.byref(h), ctypes.byref(good), 0, 1, 0) == L.ZM_ERR_DEVICE
+
+
+def test_product_does_not_import_the_oracle():
+    # The product package must never route through the checker.
+    pkg = os.path.join(ROOT, "stencilfft_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "liboracle" not in text and "oracle/" not in text, f

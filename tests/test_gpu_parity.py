@@ -1,0 +1,369 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the FP64 oracle and
+the reference's layout fixtures, at sizes the oracle finishes in seconds.
+
+Bars: data movement (permute mode, repack, index maps, layout conversion) is
+bit-exact; FP64 stencils are bit-exact (same evaluation order, no FMA
+contraction); FP64 transforms within 1e-13 relative max error; FP32 transforms
+within the 1e-6 relative-L2 budget of the FP32 path (max error 1e-5).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import d4_periodic, image_grid, lap_periodic, load_images, rel_l2, rel_max, uniform_field
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+SHAPES_2D = [(16, 2, 2), (32, 2, 4), (64, 4, 2), (64, 2, 1), (32, 2, 8), (64, 4, 4), (128, 2, 16), (256, 4, 8)]
+SHAPES_3D = [(16, 2, 2), (16, 2, 1), (32, 2, 4), (32, 4, 2)]
+PATHS = [False, True]  # fused line kernel, two-stage
+
+
+def note(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def plan(dims, n, p, b, precision="fp64", batch=1, two_stage=False):
+    from stencilfft_amd import Plan
+    return Plan(dims, n, p, b, precision=precision, batch=batch, two_stage=two_stage)
+
+
+# ---------------------------------------------------------------------------
+# data movement: bit-exact
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("two_stage", PATHS)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_permute_matches_images(precision, two_stage):
+    note(f"permute images {precision} two_stage={two_stage}")
+    imgs = load_images()
+    pl = plan(2, 16, 2, 2, precision, two_stage=two_stage)
+    x = image_grid(imgs, "step1").astype(np.complex128)
+    y = host(pl.permute_forward(dev(x, pl.complex_dtype)))
+    assert np.array_equal(y.real.astype(np.int64), image_grid(imgs, "step9"))
+    back = host(pl.permute_inverse(dev(y, pl.complex_dtype)))
+    assert np.array_equal(back.real.astype(np.int64), image_grid(imgs, "step1"))
+
+
+@pytest.mark.parametrize("two_stage", PATHS)
+@pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D])
+def test_permute_bit_exact_vs_oracle(oracle_mod, dims, n, p, b, two_stage):
+    note(f"permute {dims}D N={n} P={p} B={b} two_stage={two_stage}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    rng = np.random.default_rng(n + p + b)
+    # random bit patterns in FP64: the kernels must move bits untouched
+    bits = rng.integers(0, 2 ** 62, size=2 * s.total, dtype=np.int64).view(np.float64)
+    bits = np.where(np.isfinite(bits), bits, 1.0)
+    x = bits.view(np.complex128)
+    want = s.forward_fft(x, do_compute=False)
+    pl = plan(dims, n, p, b, "fp64", two_stage=two_stage)
+    got = host(pl.permute_forward(dev(x, torch.complex128)))
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+    back = host(pl.permute_inverse(dev(want, torch.complex128)))
+    assert np.array_equal(back.view(np.int64), x.view(np.int64))
+    # FP32: integer tags
+    pl32 = plan(dims, n, p, b, "fp32", two_stage=two_stage)
+    tags = (np.arange(s.total, dtype=np.float64) + 1j * (-np.arange(s.total, dtype=np.float64)))
+    got32 = host(pl32.permute_forward(dev(tags, torch.complex64)))
+    assert np.array_equal(got32.astype(np.complex128), s.forward_fft(tags, do_compute=False))
+
+
+@pytest.mark.parametrize("dims,n,p,b", [(2, 64, 4, 2), (2, 32, 2, 8), (3, 16, 2, 2), (3, 32, 2, 4)])
+def test_index_tables_and_layout_conversion(oracle_mod, dims, n, p, b):
+    note(f"index tables {dims}D N={n}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    pl = plan(dims, n, p, b, "fp64")
+    ramp = s.init_host()
+    assert np.array_equal(host(pl.real_index_table()), ramp.real.astype(np.int64))
+    lin = np.arange(s.total, dtype=np.float64).astype(np.complex128)
+    want_recip = s.natural_to_reciprocal(lin.reshape((n,) * dims))
+    assert np.array_equal(host(pl.reciprocal_index_table()), want_recip.real.astype(np.int64))
+    nat = uniform_field((n,) * dims, 3)
+    packed = host(pl.natural_to_packed(dev(nat.ravel(), torch.complex128)))
+    assert np.array_equal(packed, s.natural_to_real(nat))
+    back = host(pl.packed_to_natural(dev(packed, torch.complex128)))
+    assert np.array_equal(back, nat.ravel())
+
+
+# ---------------------------------------------------------------------------
+# transforms
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("two_stage", PATHS)
+@pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D])
+def test_forward_inverse_fp64_vs_oracle(oracle_mod, dims, n, p, b, two_stage):
+    note(f"fp64 fwd/inv {dims}D N={n} P={p} B={b} two_stage={two_stage}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    x = s.natural_to_real(uniform_field((n,) * dims, 100 + n + p + b))
+    pl = plan(dims, n, p, b, "fp64", two_stage=two_stage)
+    y = host(pl.forward(dev(x, torch.complex128)))
+    want = s.forward_fft(x)
+    assert rel_max(y, want) < 1e-13
+    scale = 1.0 / n ** dims
+    z = host(pl.inverse(dev(want, torch.complex128), scale))
+    assert rel_max(z, s.inverse_fft(want, scale=scale)) < 1e-13
+    assert np.max(np.abs(z - x)) < 1e-13
+
+
+@pytest.mark.parametrize("two_stage", PATHS)
+@pytest.mark.parametrize("dims,n,p,b", [(2, 64, 4, 2), (2, 256, 4, 8), (2, 128, 2, 16), (3, 32, 2, 4)])
+def test_forward_inverse_fp32_vs_oracle(oracle_mod, dims, n, p, b, two_stage):
+    note(f"fp32 fwd/inv {dims}D N={n} two_stage={two_stage}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    x = s.natural_to_real(uniform_field((n,) * dims, 200 + n)).astype(np.complex64).astype(np.complex128)
+    pl = plan(dims, n, p, b, "fp32", two_stage=two_stage)
+    y = host(pl.forward(dev(x, torch.complex64))).astype(np.complex128)
+    want = s.forward_fft(x)
+    assert rel_l2(y, want) < 1e-6 and rel_max(y, want) < 1e-5
+    scale = 1.0 / n ** dims
+    w32 = want.astype(np.complex64).astype(np.complex128)
+    z = host(pl.inverse(dev(w32, torch.complex64), scale)).astype(np.complex128)
+    zwant = s.inverse_fft(w32, scale=scale)
+    assert rel_l2(z, zwant) < 1e-6 and rel_max(z, zwant) < 1e-5
+
+
+# shapes whose (VEC, M) pairs run the compile-time specialised line pass
+FAST_SHAPES = [(2, 256, 2, 8), (2, 512, 2, 16), (2, 1024, 4, 8), (2, 2048, 2, 16), (2, 4096, 4, 32),
+               (2, 2048, 4, 4), (3, 256, 2, 8), (3, 128, 2, 4)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("dims,n,p,b", FAST_SHAPES)
+def test_fast_line_pass_vs_oracle(oracle_mod, dims, n, p, b, precision):
+    note(f"fast path {precision} {dims}D N={n} P={p} B={b}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    rng = np.random.default_rng(n + b)
+    x = (rng.uniform(-1, 1, s.total) + 1j * rng.uniform(-1, 1, s.total))
+    pl = plan(dims, n, p, b, precision)
+    cdt = pl.complex_dtype
+    if precision == "fp32":
+        x = x.astype(np.complex64).astype(np.complex128)
+    oracle_mod.set_dft_mode(1)
+    try:
+        want = s.forward_fft(x)
+        scale = 1.0 / n ** dims
+        w_in = want.astype(np.complex64).astype(np.complex128) if precision == "fp32" else want
+        zwant = s.inverse_fft(w_in, scale=scale)
+    finally:
+        oracle_mod.set_dft_mode(0)
+    y = host(pl.forward(dev(x, cdt))).astype(np.complex128)
+    z = host(pl.inverse(dev(w_in, cdt), scale)).astype(np.complex128)
+    if precision == "fp64":
+        assert rel_max(y, want) < 1e-13 and rel_max(z, zwant) < 1e-13
+    else:
+        assert rel_l2(y, want) < 1e-6 and rel_max(y, want) < 1e-5
+        assert rel_l2(z, zwant) < 1e-6 and rel_max(z, zwant) < 1e-5
+
+
+def test_config1_shape_fp64(oracle_mod):
+    # N_DIM=1024, P_DIM=2, B_DIM=16 complex128, re/im U[-1,1) -- the plain small 2D configuration
+    note("C1 N=1024 P=2 B=16 fp64")
+    n, p, b = 1024, 2, 16
+    s = oracle_mod.Shape(2, n, p, b)
+    nat = uniform_field((n, n), 1)
+    x = s.natural_to_real(nat)
+    pl = plan(2, n, p, b, "fp64")
+    y = host(pl.forward(dev(x, torch.complex128)))
+    oracle_mod.set_dft_mode(1)
+    try:
+        want = s.forward_fft(x)
+    finally:
+        oracle_mod.set_dft_mode(0)
+    assert rel_max(y, want) < 1e-13
+    assert rel_max(s.reciprocal_to_natural(y), np.fft.fft2(nat)) < 1e-13
+    z = host(pl.inverse(dev(y, torch.complex128), 1.0 / n ** 2))
+    assert np.max(np.abs(z - x)) < 1e-13
+    lap = host(pl.laplace(dev(x, torch.complex128)))
+    assert np.array_equal(lap.view(np.int64), s.laplace(x).view(np.int64))
+
+
+def test_batch_fields_are_independent(oracle_mod):
+    note("batch of 3 fields")
+    n, p, b = 64, 4, 2
+    s = oracle_mod.Shape(2, n, p, b)
+    xs = [s.natural_to_real(uniform_field((n, n), 300 + k)) for k in range(3)]
+    pl = plan(2, n, p, b, "fp64", batch=3)
+    y = host(pl.forward(dev(np.concatenate(xs), torch.complex128))).reshape(3, -1)
+    for k in range(3):
+        assert rel_max(y[k], s.forward_fft(xs[k])) < 1e-13
+
+
+def test_ramp_closed_form_on_gpu(oracle_mod):
+    from helpers import ramp_dft2
+    note("ramp closed form")
+    for n, p, b in [(64, 4, 2), (256, 4, 8)]:
+        s = oracle_mod.Shape(2, n, p, b)
+        pl = plan(2, n, p, b, "fp64")
+        y = host(pl.forward(dev(s.init_host(), torch.complex128)))
+        assert rel_max(s.reciprocal_to_natural(y), ramp_dft2(n)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# stencils
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D])
+def test_laplace_fp64_bit_exact(oracle_mod, dims, n, p, b):
+    note(f"laplace {dims}D N={n} P={p} B={b}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    x = s.natural_to_real(uniform_field((n,) * dims, 400 + n))
+    pl = plan(dims, n, p, b, "fp64")
+    got = host(pl.laplace(dev(x, torch.complex128)))
+    assert np.array_equal(got.view(np.int64), s.laplace(x).view(np.int64))
+    # real elements and a scale
+    xr = np.ascontiguousarray(x.real)
+    gotr = host(pl.laplace(dev(xr, torch.float64), scale=float(n * n)))
+    assert np.allclose(gotr, s.laplace(xr) * n * n, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("n,p,b", [(16, 2, 2), (32, 2, 4), (64, 4, 2), (32, 2, 8), (64, 2, 2), (256, 4, 8)])
+def test_dx_dy_fp64_bit_exact(oracle_mod, n, p, b):
+    note(f"dx/dy N={n} P={p} B={b}")
+    s = oracle_mod.Shape(2, n, p, b)
+    rng = np.random.default_rng(500 + n)
+    xr = s.natural_to_real(rng.uniform(-1, 1, (n, n)))
+    xc = s.natural_to_real(uniform_field((n, n), 501 + n))
+    pl = plan(2, n, p, b, "fp64")
+    for delta in (1.0, 1.0 / n, 0.3):
+        for x, dt in ((xr, torch.float64), (xc, torch.complex128)):
+            gx = host(pl.dx4(dev(x, dt), delta))
+            gy = host(pl.dy4(dev(x, dt), delta))
+            assert np.array_equal(gx.view(np.int64), s.dx(x, delta).view(np.int64))
+            assert np.array_equal(gy.view(np.int64), s.dy(x, delta).view(np.int64))
+            ax = host(pl.dx4(dev(x, dt), delta, adjoint=True))
+            assert np.array_equal(ax, -gx)
+
+
+@pytest.mark.parametrize("dims,n,p,b", [(2, 256, 4, 8), (3, 32, 2, 4)])
+def test_stencils_fp32(oracle_mod, dims, n, p, b):
+    note(f"fp32 stencils {dims}D N={n}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    x = s.natural_to_real(uniform_field((n,) * dims, 600)).astype(np.complex64)
+    pl = plan(dims, n, p, b, "fp32")
+    got = host(pl.laplace(dev(x, torch.complex64))).astype(np.complex128)
+    want = s.laplace(x.astype(np.complex128))
+    assert rel_l2(got, want) < 1e-6
+    if dims == 2:
+        got = host(pl.dx4(dev(x, torch.complex64), 1.0 / n)).astype(np.complex128)
+        want = s.dx(x.astype(np.complex128), 1.0 / n)
+        assert rel_l2(got, want) < 1e-6
+
+
+def test_stencil_natural_layout_definitions(oracle_mod):
+    note("stencils vs periodic definitions")
+    n, p, b = 64, 4, 4
+    s = oracle_mod.Shape(2, n, p, b)
+    nat = uniform_field((n, n), 700)
+    pl = plan(2, n, p, b, "fp64")
+    x = dev(s.natural_to_real(nat), torch.complex128)
+    assert np.max(np.abs(s.real_to_natural(host(pl.laplace(x))) - lap_periodic(nat))) < 1e-13
+    assert np.max(np.abs(s.real_to_natural(host(pl.dx4(x, 0.5))) - d4_periodic(nat, 1, 0.5))) < 1e-13
+    assert np.max(np.abs(s.real_to_natural(host(pl.dy4(x, 0.5))) - d4_periodic(nat, 0, 0.5))) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# adjoints and spectral multipliers
+# ---------------------------------------------------------------------------
+
+def _dot(a, b):
+    return np.vdot(b, a)  # <a, b> = sum conj(b) a
+
+
+@pytest.mark.parametrize("dims,n,p,b", [(2, 64, 4, 2), (3, 16, 2, 2)])
+def test_adjoint_identities(oracle_mod, dims, n, p, b):
+    note(f"adjoints {dims}D")
+    s = oracle_mod.Shape(dims, n, p, b)
+    pl = plan(dims, n, p, b, "fp64")
+    x = s.natural_to_real(uniform_field((n,) * dims, 800))
+    y = s.natural_to_real(uniform_field((n,) * dims, 801))
+    Fx = host(pl.forward(dev(x, torch.complex128)))
+    FHy = host(pl.forward_adjoint(dev(y, torch.complex128)))
+    assert abs(_dot(Fx, y) - _dot(x, FHy)) < 1e-10 * abs(_dot(Fx, y))
+    sc = 0.37
+    Ix = host(pl.inverse(dev(x, torch.complex128), sc))
+    IHy = host(pl.inverse_adjoint(dev(y, torch.complex128), sc))
+    assert abs(_dot(Ix, y) - _dot(x, IHy)) < 1e-10 * abs(_dot(Ix, y))
+    Lx = host(pl.laplace(dev(x, torch.complex128)))
+    Ly = host(pl.laplace(dev(y, torch.complex128)))
+    assert abs(_dot(Lx, y) - _dot(x, Ly)) < 1e-10 * abs(_dot(Lx, y))
+    if dims == 2:
+        Dx = host(pl.dx4(dev(x, torch.complex128), 0.25))
+        DHy = host(pl.dx4(dev(y, torch.complex128), 0.25, adjoint=True))
+        assert abs(_dot(Dx, y) - _dot(x, DHy)) < 1e-10 * abs(_dot(Dx, y))
+        Dy = host(pl.dy4(dev(x, torch.complex128), 0.25))
+        DyH = host(pl.dy4(dev(y, torch.complex128), 0.25, adjoint=True))
+        assert abs(_dot(Dy, y) - _dot(x, DyH)) < 1e-10 * abs(_dot(Dy, y))
+
+
+@pytest.mark.parametrize("dims,n,p,b", [(2, 64, 4, 2), (3, 32, 2, 4)])
+def test_spectral_multipliers(oracle_mod, dims, n, p, b):
+    note(f"spectral {dims}D")
+    s = oracle_mod.Shape(dims, n, p, b)
+    pl = plan(dims, n, p, b, "fp64")
+    x = uniform_field(s.total, 900).astype(np.complex128)
+    k = np.fft.fftfreq(n, 1.0 / n)
+    grids = np.meshgrid(*([k] * dims), indexing="ij")
+    k2 = sum(g ** 2 for g in grids) * (2 * np.pi) ** 2
+    k2_grid = s.natural_to_reciprocal(k2.astype(np.complex128)).real
+    pois = np.where(k2_grid == 0, 0.0, -1.0 / np.where(k2_grid == 0, 1.0, k2_grid))
+    got = host(pl.spectral_multiply(dev(x, torch.complex128), "poisson"))
+    assert np.allclose(got, x * pois, rtol=1e-15, atol=0)
+    got = host(pl.spectral_multiply(dev(x, torch.complex128), "heat", 1e-3, 1e-2))
+    assert np.allclose(got, x * np.exp(-1e-3 * k2_grid * 1e-2), rtol=1e-14, atol=0)
+
+
+def test_poisson_solve_recovers_laplacian_inverse(oracle_mod):
+    """u = IFFT(FFT(f) / (-|k|^2)) solves the spectral Poisson equation: FFT(u) * (-|k|^2) = FFT(f) - mean."""
+    note("poisson solve")
+    n, p, b = 64, 4, 2
+    s = oracle_mod.Shape(2, n, p, b)
+    pl = plan(2, n, p, b, "fp64")
+    gr, gc = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    f = np.sin(2 * np.pi * 3 * gc / n) * np.cos(2 * np.pi * 5 * gr / n) + 0j
+    x = dev(s.natural_to_real(f), torch.complex128)
+    u = pl.inverse(pl.spectral_multiply(pl.forward(x), "poisson"), 1.0 / n ** 2)
+    un = s.real_to_natural(host(u))
+    want = -f / ((2 * np.pi) ** 2 * (3 ** 2 + 5 ** 2))
+    assert np.max(np.abs(un - want)) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# error paths
+# ---------------------------------------------------------------------------
+
+def test_error_paths():
+    from stencilfft_amd import ZmError
+    note("error paths")
+    pl = plan(2, 64, 2, 2, "fp64")
+    x = torch.zeros(64 * 64, dtype=torch.complex128, device="cuda")
+    with pytest.raises(ZmError) as e:
+        pl.forward(x, out=x)
+    assert e.value.status == 5
+    with pytest.raises(ValueError):
+        pl.forward(torch.zeros(64 * 64, dtype=torch.complex64, device="cuda"))
+    p1 = plan(2, 64, 2, 1, "fp64")
+    with pytest.raises(ZmError) as e:
+        p1.dx4(torch.zeros(64 * 64, dtype=torch.float64, device="cuda"))
+    assert e.value.status == 3
+    with pytest.raises(ZmError) as e:
+        plan(2, 64, 2, 2, "fp32_mfma")
+    assert e.value.status == 3
