@@ -165,6 +165,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=8192)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain device-to-device copy of one N^2 complex64 field (the HBM reference "
+                         "kernel the rocprof FETCH_SIZE / WRITE_SIZE counters are scaled by) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -181,6 +184,26 @@ def main():
         sys.exit(1)
 
     n, p, b = args.n, args.p, args.b
+    if args.copy_calibration:
+        src = torch.ones(n * n, dtype=torch.complex64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        ev0.record()
+        for _ in range(reps):
+            dst.copy_(src)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        nbytes = 2 * src.numel() * src.element_size()
+        print(json.dumps({"metric": "device-to-device copy bandwidth", "value": round(nbytes / ms / 1e6, 1),
+                          "unit": "GB/s", "bytes_per_copy_read": src.numel() * src.element_size(),
+                          "bytes_per_copy_write": src.numel() * src.element_size(), "ms_per_copy": round(ms, 4),
+                          "frac_of_peak": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}))
+        return
     log(f"workload 2D N={n} P={p} B={b} precision={args.precision} steps={args.steps} warmup={args.warmup}")
     nat = mode_field_gpu(n, 256, n // 8, 1e-3, args.seed, torch)
     plan64 = Plan(2, n, p, b, precision="fp64", two_stage=args.two_stage)
