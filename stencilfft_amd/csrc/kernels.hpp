@@ -51,10 +51,6 @@ hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const Repack
 template <typename T>
 bool fast_line_supported(int vec, int m);
 template <typename T>
-size_t fast_line_lds_bytes(int vec, int m);
-template <typename T>
-int fast_line_lines_per_iter(int vec, int m);
-template <typename T>
 int fast_line_occupancy(int vec, int m, bool fwd);
 template <typename T>
 hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,

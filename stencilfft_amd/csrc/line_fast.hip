@@ -71,7 +71,7 @@ __device__ __forceinline__ C tw_n(const C* ta, const C* tb, int e) {
 // PRE: multiply element i of transform bt by W_N^(i * kk(bt)) first.
 // TOGLOBAL: instead of writing LDS, hand each output to `sink`.
 template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool TOGLOBAL, typename Sink>
+          bool TOGLOBAL, bool FULLTAB, typename Sink>
 __device__ __forceinline__ void fast_pass(Cx<T>* lds, const Cx<T>* ts, int ls, const Cx<T>* ta, const Cx<T>* tb,
                                           int tid, Sink&& sink) {
     using C = Cx<T>;
@@ -109,7 +109,9 @@ __device__ __forceinline__ void fast_pass(Cx<T>* lds, const Cx<T>* ts, int ls, c
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int e = ((j + r * nbf) * kk) & (N - 1);
-                v[k][r] = ctw<FWD>(v[k][r], cmul(ta[e & 63], tb[e >> 6]));
+                // FULLTAB: ta is the whole W_N table; otherwise W_N^e = A[e & 63] * B[e >> 6]
+                if constexpr (FULLTAB) v[k][r] = ctw<FWD>(v[k][r], ta[e]);
+                else v[k][r] = ctw<FWD>(v[k][r], cmul(ta[e & 63], tb[e >> 6]));
             }
         }
         if (NS > 1) {
@@ -147,22 +149,24 @@ struct NoSink {
     __device__ void operator()(int, int, C) const {}
 };
 
-// Run passes FIRST..LAST-1 of a length-2^LN_T transform (all in LDS).
-template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE>
+// Run passes PASS..NPASS-1 of a length-2^LN_T transform (all in LDS).
+template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
+          bool FULLTAB>
 __device__ __forceinline__ void fast_passes_lds(Cx<T>* lds, const Cx<T>* ts, int ls, const Cx<T>* ta,
                                                 const Cx<T>* tb, int tid) {
     if constexpr (PASS < NPASS) {
         constexpr int R = fast::Radix<LN_T, PASS>::value;
         constexpr int NS = 1 << (4 * PASS);
-        fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false>(lds, ts, ls, ta, tb, tid,
-                                                                                           NoSink{});
-        fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE>(lds, ts, ls, ta, tb, tid);
+        fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB>(lds, ts, ls, ta, tb,
+                                                                                                    tid, NoSink{});
+        fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB>(lds, ts, ls, ta, tb,
+                                                                                                tid);
     }
 }
 
 // Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
 // iteration.  a.lpb is the number of iterations (line groups) per workgroup.
-template <typename T, int LVEC, int LM, int LLPI, bool FWD>
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB>
 __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     line_fast_kernel(const Cx<T>* in, Cx<T>* out, Geo g, LineArgs a, const Cx<T>* __restrict__ tables, T scale,
                      int apply_scale, int groups) {
@@ -173,13 +177,19 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     constexpr int NT = E / EPT;
     constexpr int LS = LVEC > LM ? LVEC : LM, S = 1 << LS;
     constexpr int NPV = fast::NumPasses<LVEC>::value, NPM = fast::NumPasses<LM>::value;
-    constexpr int NTAB = S + 64 + (N / 64 > 0 ? N / 64 : 1);
+    constexpr int NTWO = S + 64 + (N / 64 > 0 ? N / 64 : 1);  // W_S + two-level W_N
     extern __shared__ unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     C* tabs = lds + padded_elems(E);
     const int tid = threadIdx.x;
 
-    for (int t = tid; t < NTAB; t += NT) tabs[t] = tables[t];
+    // global table layout: [W_S | A (64) | B (N/64) | W_N (N)]
+    if constexpr (FULLTAB) {
+        for (int t = tid; t < S; t += NT) tabs[t] = tables[t];
+        for (int t = tid; t < N; t += NT) tabs[S + t] = tables[NTWO + t];
+    } else {
+        for (int t = tid; t < NTWO; t += NT) tabs[t] = tables[t];
+    }
     const C* ts = tabs;
     const C* ta = tabs + S;
     const C* tb = tabs + S + 64;
@@ -243,14 +253,14 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
 
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
-            fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false>(lds, ts, LS, ta, tb, tid);
+            fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB>(lds, ts, LS, ta, tb, tid);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
-            fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true>(lds, ts, LS, ta, tb, tid);
+            fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB>(lds, ts, LS, ta, tb, tid);
             constexpr int PL = NPM - 1;
             constexpr int RL = fast::Radix<LM, PL>::value;
             constexpr int NSL = 1 << (4 * PL);
-            fast_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, true>(
+            fast_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, true, FULLTAB>(
                 lds, ts, LS, ta, tb, tid, [&](int u, int k2, C v) {
                     // transform (line, k1): k1 = u & (VEC-1), line = u >> (LVEC + log2(nbf))
                     constexpr int nbf = (1 << LM) / RL;
@@ -262,13 +272,14 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         } else {
             // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
             // the conjugate inter-stage twiddle; the last pass stores to HBM
-            fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false>(lds, ts, LS, ta, tb, tid);
-            fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true>(lds, ts, LS, ta, tb, tid);
+            fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB>(lds, ts, LS, ta, tb, tid);
+            fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB>(lds, ts, LS, ta, tb,
+                                                                                                tid);
             constexpr int PL = NPV - 1;
             constexpr int RL = fast::Radix<LVEC, PL>::value;
             constexpr int NSL = 1 << (4 * PL);
             const T sc = scale;
-            fast_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, true>(
+            fast_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, true, FULLTAB>(
                 lds, ts, LS, ta, tb, tid, [&](int u, int jj, C v) {
                     // transform (line, r): r = bt & (M-1), line = bt >> LM
                     constexpr int nbf = (1 << LVEC) / RL;
@@ -285,75 +296,127 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
 
 // host side ------------------------------------------------------------------
 
-template <typename T, int LVEC, int LM, int LLPI, bool FWD>
+namespace {
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+int lines_per_iter_log2(int n) {
+    int l = 0;
+    while ((n << l) < 4096) l++;
+    return l;
+}
+
+// LDS bytes of the specialised kernel for element size `es`
+size_t fast_lds(size_t es, int vec, int m, bool fulltab) {
+    const int n = vec * m, s = vec > m ? vec : m;
+    const int e = n << lines_per_iter_log2(n);
+    const int tab = fulltab ? s + n : s + 64 + (n / 64 > 0 ? n / 64 : 1);
+    return es * (padded_elems(e) + tab);
+}
+
+}  // namespace
+
+template <typename T>
+bool fast_line_fulltab(int vec, int m) {
+    return fast_lds(sizeof(Cx<T>), vec, m, true) <= kLdsLimit;
+}
+
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB>
+static const void* fast_fn() {
+    return (const void*)&line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB>;
+}
+
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB>
 static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a,
                                    const Cx<T>* tables, int batch, double scale, int grid_x, hipStream_t stream) {
     constexpr int LN = LVEC + LM, N = 1 << LN, E = N << LLPI;
-    constexpr int S = (1 << (LVEC > LM ? LVEC : LM));
-    constexpr int NTAB = S + 64 + (N / 64 > 0 ? N / 64 : 1);
-    const size_t lds = sizeof(Cx<T>) * (padded_elems(E) + NTAB);
-    const void* fn = (const void*)&line_fast_kernel<T, LVEC, LM, LLPI, FWD>;
+    const size_t lds = fast_lds(sizeof(Cx<T>), 1 << LVEC, 1 << LM, FULLTAB);
+    const void* fn = fast_fn<T, LVEC, LM, LLPI, FWD, FULLTAB>();
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int groups = (int)(g.field / N) >> LLPI;
     const dim3 grid((unsigned)(grid_x < groups ? grid_x : groups), (unsigned)batch);
-    hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD>), grid, dim3(E / EPT), lds, stream, in, out, g, a,
-                       tables, (T)scale, scale != 1.0 ? 1 : 0, groups);
+    hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB>), grid, dim3(E / EPT), lds, stream, in, out,
+                       g, a, tables, (T)scale, scale != 1.0 ? 1 : 0, groups);
     return hipGetLastError();
 }
 
-template <typename T, int LVEC, int LM>
-static hipError_t launch_fast_lm(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
-                                 int batch, bool fwd, double scale, int grid_x, hipStream_t stream) {
-    // lines per iteration: E = N * LPI ~ 4096 elements, at least one line
-    constexpr int LN = LVEC + LM;
-    constexpr int LLPI = LN >= 12 ? 0 : 12 - LN;
-    if (fwd) return launch_fast_inst<T, LVEC, LM, LLPI, true>(in, out, g, a, tables, batch, scale, grid_x, stream);
-    return launch_fast_inst<T, LVEC, LM, LLPI, false>(in, out, g, a, tables, batch, scale, grid_x, stream);
-}
-
-template <typename T>
-bool fast_line_supported(int vec, int m) {
-    const bool okv = vec == 16 || vec == 32 || vec == 64 || vec == 128;
-    const bool okm = m == 16 || m == 32 || m == 64 || m == 128;
-    return okv && okm && (int64_t)vec * m <= 8192;
-}
-
-template <typename T>
-size_t fast_line_lds_bytes(int vec, int m) {
-    const int n = vec * m;
-    int llpi = 0;
-    while ((n << llpi) < 4096) llpi++;
-    const int s = vec > m ? vec : m;
-    return sizeof(Cx<T>) * (padded_elems(n << llpi) + s + 64 + (n / 64 > 0 ? n / 64 : 1));
-}
-
-template <typename T>
-int fast_line_lines_per_iter(int vec, int m) {
-    int llpi = 0;
-    while ((vec * m << llpi) < 4096) llpi++;
-    return 1 << llpi;
-}
-
-template <typename T>
-hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
-                            int batch, bool fwd, double scale, int grid_x, hipStream_t stream) {
-#define ZM_LF(V, MM)                                                                                    \
-    if (g.vec == (1 << V) && g.m == (1 << MM))                                                          \
-        return launch_fast_lm<T, V, MM>(in, out, g, a, tables, batch, fwd, scale, grid_x, stream);
+// Dispatch over the instantiated (VEC, M) pairs; `op` is called with the
+// kernel's template arguments.
+template <typename T, typename Op>
+static auto with_fast(int vec, int m, bool fwd, Op&& op) {
+#define ZM_LF1(V, MM, F, FT)                                                          \
+    if (vec == (1 << V) && m == (1 << MM) && fwd == F && fast_line_fulltab<T>(vec, m) == FT) \
+        return op.template run<V, MM, ((V + MM) >= 12 ? 0 : 12 - (V + MM)), F, FT>();
+#define ZM_LF(V, MM) ZM_LF1(V, MM, true, true) ZM_LF1(V, MM, true, false) ZM_LF1(V, MM, false, true) ZM_LF1(V, MM, false, false)
     ZM_LF(4, 4) ZM_LF(4, 5) ZM_LF(4, 6) ZM_LF(4, 7)
     ZM_LF(5, 4) ZM_LF(5, 5) ZM_LF(5, 6) ZM_LF(5, 7)
     ZM_LF(6, 4) ZM_LF(6, 5) ZM_LF(6, 6) ZM_LF(6, 7)
     ZM_LF(7, 4) ZM_LF(7, 5) ZM_LF(7, 6)
 #undef ZM_LF
-    return hipErrorInvalidValue;
+#undef ZM_LF1
+    return op.none();
 }
 
-// FP64 twiddle tables of a line (length N = vec*m), rounded to T: see TwTables.
+template <typename T>
+struct LaunchOp {
+    const Cx<T>* in;
+    Cx<T>* out;
+    const Geo& g;
+    const LineArgs& a;
+    const Cx<T>* tables;
+    int batch;
+    double scale;
+    int grid_x;
+    hipStream_t stream;
+    template <int V, int MM, int LL, bool F, bool FT>
+    hipError_t run() {
+        return launch_fast_inst<T, V, MM, LL, F, FT>(in, out, g, a, tables, batch, scale, grid_x, stream);
+    }
+    hipError_t none() { return hipErrorInvalidValue; }
+};
+
+template <typename T>
+struct OccOp {
+    int vec, m;
+    template <int V, int MM, int LL, bool F, bool FT>
+    int run() {
+        const void* fn = fast_fn<T, V, MM, LL, F, FT>();
+        const size_t lds = fast_lds(sizeof(Cx<T>), vec, m, FT);
+        const int nthreads = ((1 << (V + MM)) << LL) / EPT;
+        if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, nthreads, lds) != hipSuccess) return 0;
+        return blocks;
+    }
+    int none() { return 0; }
+};
+
+template <typename T>
+bool fast_line_supported(int vec, int m) {
+    const bool okv = vec == 16 || vec == 32 || vec == 64 || vec == 128;
+    const bool okm = m == 16 || m == 32 || m == 64 || m == 128;
+    return okv && okm && (int64_t)vec * m <= 8192 && fast_lds(sizeof(Cx<T>), vec, m, false) <= kLdsLimit;
+}
+
+template <typename T>
+int fast_line_occupancy(int vec, int m, bool fwd) {
+    return with_fast<T>(vec, m, fwd, OccOp<T>{vec, m});
+}
+
+template <typename T>
+hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
+                            int batch, bool fwd, double scale, int grid_x, hipStream_t stream) {
+    return with_fast<T>(g.vec, g.m, fwd, LaunchOp<T>{in, out, g, a, tables, batch, scale, grid_x, stream});
+}
+
+// FP64 twiddle tables of a line (N = vec*m, S = max(vec, m)), rounded for
+// FP32:  [W_S^t, t < S | W_N^t, t < 64 | W_N^(64 t), t < N/64 | W_N^t, t < N]
 __global__ void line_tables_kernel(double2* t64, float2* t32, int n, int s) {
-    const int nt = s + 64 + (n / 64 > 0 ? n / 64 : 1);
+    const int nb = n / 64 > 0 ? n / 64 : 1;
+    const int nt = s + 64 + nb + n;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     double num, den;
@@ -363,8 +426,11 @@ __global__ void line_tables_kernel(double2* t64, float2* t32, int n, int s) {
     } else if (t < s + 64) {
         num = t - s;
         den = n;
-    } else {
+    } else if (t < s + 64 + nb) {
         num = 64.0 * (t - s - 64);
+        den = n;
+    } else {
+        num = t - s - 64 - nb;
         den = n;
     }
     double sn, cs;
@@ -375,7 +441,7 @@ __global__ void line_tables_kernel(double2* t64, float2* t32, int n, int s) {
 
 int line_table_entries(int vec, int m) {
     const int n = vec * m, s = vec > m ? vec : m;
-    return s + 64 + (n / 64 > 0 ? n / 64 : 1);
+    return s + 64 + (n / 64 > 0 ? n / 64 : 1) + n;
 }
 
 hipError_t launch_line_tables(double2* t64, float2* t32, int vec, int m, hipStream_t stream) {
@@ -385,35 +451,8 @@ hipError_t launch_line_tables(double2* t64, float2* t32, int vec, int m, hipStre
     return hipGetLastError();
 }
 
-template <typename T>
-int fast_line_occupancy(int vec, int m, bool fwd) {
-    int blocks = 0;
-    const size_t lds = fast_line_lds_bytes<T>(vec, m);
-    const int nthreads = vec * m * fast_line_lines_per_iter<T>(vec, m) / EPT;
-    const void* fn = nullptr;
-#define ZM_LF(V, MM)                                                                                       \
-    if (vec == (1 << V) && m == (1 << MM)) {                                                               \
-        constexpr int LL = (V + MM) >= 12 ? 0 : 12 - (V + MM);                                             \
-        fn = fwd ? (const void*)&line_fast_kernel<T, V, MM, LL, true>                                      \
-                 : (const void*)&line_fast_kernel<T, V, MM, LL, false>;                                    \
-    }
-    ZM_LF(4, 4) ZM_LF(4, 5) ZM_LF(4, 6) ZM_LF(4, 7)
-    ZM_LF(5, 4) ZM_LF(5, 5) ZM_LF(5, 6) ZM_LF(5, 7)
-    ZM_LF(6, 4) ZM_LF(6, 5) ZM_LF(6, 6) ZM_LF(6, 7)
-    ZM_LF(7, 4) ZM_LF(7, 5) ZM_LF(7, 6)
-#undef ZM_LF
-    if (!fn) return 0;
-    if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, nthreads, lds) != hipSuccess) return 0;
-    return blocks;
-}
-
 template bool fast_line_supported<float>(int, int);
 template bool fast_line_supported<double>(int, int);
-template size_t fast_line_lds_bytes<float>(int, int);
-template size_t fast_line_lds_bytes<double>(int, int);
-template int fast_line_lines_per_iter<float>(int, int);
-template int fast_line_lines_per_iter<double>(int, int);
 template int fast_line_occupancy<float>(int, int, bool);
 template int fast_line_occupancy<double>(int, int, bool);
 template hipError_t launch_line_fast<float>(const float2*, float2*, const Geo&, const LineArgs&, const float2*, int,

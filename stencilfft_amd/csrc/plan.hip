@@ -216,8 +216,7 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
     // the specialised line pass, when the (VEC, M) pair has one and its LDS fits
     const bool fp64 = precision == ZM_FP64;
     if (e == hipSuccess && !two_stage &&
-        (fp64 ? zm::fast_line_supported<double>(d.vec, d.m) : zm::fast_line_supported<float>(d.vec, d.m)) &&
-        (fp64 ? zm::fast_line_lds_bytes<double>(d.vec, d.m) : zm::fast_line_lds_bytes<float>(d.vec, d.m)) <= kMaxLds) {
+        (fp64 ? zm::fast_line_supported<double>(d.vec, d.m) : zm::fast_line_supported<float>(d.vec, d.m))) {
         const int nt = zm::line_table_entries(d.vec, d.m);
         if (hipMalloc(&p->ltab64, sizeof(double2) * nt) == hipSuccess &&
             hipMalloc(&p->ltab32, sizeof(float2) * nt) == hipSuccess) {

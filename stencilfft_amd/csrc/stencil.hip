@@ -190,6 +190,101 @@ __global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E
     out[(int64_t)blockIdx.y * g.field + i] = s;
 }
 
+// 2D Laplacian as a rolling strip: a workgroup owns one column rank cid and
+// T consecutive global rows gr; every thread keeps the rows above, at and
+// below its K = L/256 columns in registers, so each row is read from HBM
+// once per strip.  The current row is staged in LDS with a VEC-wide halo on
+// each side (the neighbour ranks' edge vectors, lane-shifted at the global x
+// edge as laplace.cu:148,160 do), so left = lds[pos] and right = lds[pos + 2 VEC].
+template <typename E, typename S, int K>
+__global__ void __launch_bounds__(256) laplace2d_strip_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g,
+                                                              int rows_per_strip, S scale, int apply_scale) {
+#pragma clang fp contract(off)
+    using O = EOps<E>;
+    extern __shared__ unsigned char smem_raw[];
+    E* row = reinterpret_cast<E*>(smem_raw);  // [VEC halo | L | VEC halo]
+    const int tid = threadIdx.x;
+    const int N = g.n, L = g.l, VEC = g.vec, B = g.b, P = g.p;
+    const int cid = blockIdx.y;
+    const int g0 = blockIdx.x * rows_per_strip;
+    const int64_t fbase = (int64_t)blockIdx.z * g.field;
+    const E* src = in + fbase;
+    E* dst = out + fbase;
+    const int cl = (cid - 1 + P) & (P - 1), cr = (cid + 1) & (P - 1);
+    // offset of global row gr of column rank c
+    auto row_off = [&](int gr, int c) -> int64_t {
+        gr &= N - 1;
+        const int rid = (gr >> g.lb) & (P - 1);
+        const int lr = ((gr >> (g.lb + g.lp)) << g.lb) + (gr & (B - 1));
+        return (int64_t)((rid << g.lp) + c) * g.local + (int64_t)lr * L;
+    };
+    E up[K], ce[K], dn[K];
+    {
+        const int64_t ou = row_off(g0 - 1, cid), oc = row_off(g0, cid);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            up[k] = src[ou + tid + 256 * k];
+            ce[k] = src[oc + tid + 256 * k];
+        }
+    }
+    for (int t = 0; t < rows_per_strip; t++) {
+        const int gr = g0 + t;
+        const int64_t od = row_off(gr + 1, cid);
+#pragma unroll
+        for (int k = 0; k < K; k++) dn[k] = src[od + tid + 256 * k];
+        // stage the current row and its halos
+#pragma unroll
+        for (int k = 0; k < K; k++) row[VEC + tid + 256 * k] = ce[k];
+        for (int jj = tid; jj < VEC; jj += 256) {
+            const int jl = (cid == 0) ? ((jj - 1 + VEC) & (VEC - 1)) : jj;
+            const int jr = (cid == P - 1) ? ((jj + 1) & (VEC - 1)) : jj;
+            row[jj] = src[row_off(gr, cl) + (B - 1) * VEC + jl];
+            row[VEC + L + jj] = src[row_off(gr, cr) + jr];
+        }
+        __syncthreads();
+        const int64_t oc = row_off(gr, cid);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int pos = tid + 256 * k;
+            E acc = O::zero();
+            acc = O::add(acc, up[k]);
+            acc = O::add(acc, dn[k]);
+            acc = O::add(acc, row[pos]);
+            acc = O::add(acc, row[pos + 2 * VEC]);
+            acc = O::sub(acc, O::mul(ce[k], (S)4));
+            if (apply_scale) acc = O::mul(acc, scale);
+            dst[oc + pos] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            up[k] = ce[k];
+            ce[k] = dn[k];
+        }
+    }
+}
+
+template <typename E, typename S>
+static hipError_t laplace2d_strip(const E* in, E* out, const Geo& g, int batch, double coef, hipStream_t stream) {
+    const int rows = 64 < g.n ? 64 : g.n;
+    const dim3 grid((unsigned)(g.n / rows), (unsigned)g.p, (unsigned)batch);
+    const size_t lds = sizeof(E) * (size_t)(g.l + 2 * g.vec);
+    const S sc = (S)coef;
+    const int apply = coef != 1.0 ? 1 : 0;
+    switch (g.l) {
+        case 256: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 1>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
+        case 512: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 2>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
+        case 1024: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 4>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
+        case 2048: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 8>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+static bool strip_ok(const Geo& g) {
+    return g.dims == 2 && (g.l == 256 || g.l == 512 || g.l == 1024 || g.l == 2048);
+}
+
 template <typename E, typename S>
 static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int batch, StencilOp op, double coef,
                                    bool negate, hipStream_t stream) {
@@ -197,7 +292,9 @@ static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int 
     const dim3 grid((unsigned)((g.field + threads - 1) / threads), (unsigned)batch);
     const E* src = static_cast<const E*>(in);
     E* dst = static_cast<E*>(out);
-    if (op == OP_LAPLACE) {
+    if (op == OP_LAPLACE && strip_ok(g)) {
+        return laplace2d_strip<E, S>(src, dst, g, batch, coef, stream);
+    } else if (op == OP_LAPLACE) {
         hipLaunchKernelGGL((laplace_kernel<E, S>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
                            coef != 1.0 ? 1 : 0);
     } else if (op == OP_DX4) {

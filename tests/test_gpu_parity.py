@@ -221,7 +221,11 @@ def test_ramp_closed_form_on_gpu(oracle_mod):
 # stencils
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D])
+STRIP_SHAPES = [(2, 512, 2, 16), (2, 1024, 4, 8), (2, 1024, 2, 8), (2, 2048, 2, 16), (2, 4096, 2, 32),
+                (2, 8192, 4, 64)]
+
+
+@pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D] + STRIP_SHAPES)
 def test_laplace_fp64_bit_exact(oracle_mod, dims, n, p, b):
     note(f"laplace {dims}D N={n} P={p} B={b}")
     s = oracle_mod.Shape(dims, n, p, b)
