@@ -36,35 +36,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def mode_field_gpu(n, nmodes, kmax, noise, seed, torch):
-    """Natural-layout complex128 field: exact sum of `nmodes` plane waves
-    (evaluated as a rank-`nmodes` complex GEMM in FP64) plus uniform noise."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    ks, coef = [], []
-    while len(ks) < nmodes:
-        k = rng.integers(-kmax, kmax + 1, size=2)
-        k2 = float(k[0] ** 2 + k[1] ** 2)
-        if k2 > kmax * kmax:
-            continue
-        ks.append(k)
-        coef.append(rng.normal() / (1.0 + k2) * np.exp(1j * rng.uniform(0, 2 * np.pi)))
-    ks = np.array(ks)
-    pos = np.arange(n)
-    # exact integer phase reduction: (k * x) mod n
-    uy = np.exp(2j * np.pi * ((ks[:, 0][None, :] * pos[:, None]) % n) / n)  # (n, m): rows gr
-    vx = np.exp(2j * np.pi * ((ks[:, 1][None, :] * pos[:, None]) % n) / n)  # (n, m): cols gc
-    U = torch.from_numpy(uy * np.array(coef)[None, :]).cuda()
-    V = torch.from_numpy(np.ascontiguousarray(vx.T)).cuda()
-    f = U @ V
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    re = torch.rand((n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
-    im = torch.rand((n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
-    f += noise * torch.complex(re, im)
-    return f.reshape(-1).contiguous()
-
-
 def rel_errors(a, ref, torch):
     d = (a.to(torch.complex128) - ref)
     l2 = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
@@ -205,7 +176,8 @@ def main():
                           "frac_of_peak": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}))
         return
     log(f"workload 2D N={n} P={p} B={b} precision={args.precision} steps={args.steps} warmup={args.warmup}")
-    nat = mode_field_gpu(n, 256, n // 8, 1e-3, args.seed, torch)
+    from stencilfft_amd.fields import mode_field_natural
+    nat = mode_field_natural(n, 2, 256, n // 8, args.seed, noise=1e-3)
     plan64 = Plan(2, n, p, b, precision="fp64", two_stage=args.two_stage)
     x64 = plan64.natural_to_packed(nat)
     del nat

@@ -164,17 +164,25 @@ zm_status zm_dy4_adjoint(zm_plan* plan, const void* in, void* out, double delta,
 /* ---------------------------------------------------------------------- */
 
 typedef enum {
-    ZM_SPECTRAL_POISSON = 0, /* 1 / (-|k|^2), k = 0 mode set to 0 */
-    ZM_SPECTRAL_HEAT = 1,    /* exp(-nu |k|^2 dt): p0 = nu, p1 = dt */
-    ZM_SPECTRAL_SCALE = 2    /* constant p0 */
+    ZM_SPECTRAL_POISSON = 0,        /* 1 / (-|k|^2), k = 0 mode set to 0 */
+    ZM_SPECTRAL_HEAT = 1,           /* exp(-nu |k|^2 dt): p0 = nu, p1 = dt */
+    ZM_SPECTRAL_SCALE = 2,          /* constant p0 */
+    ZM_SPECTRAL_STENCIL = 3,        /* exact symbol of zm_laplace(scale = p0) + zm_dx4(delta = p1);
+                                       p1 = 0 drops the dx4 term (which is 2D only) */
+    ZM_SPECTRAL_POISSON_STENCIL = 4 /* POISSON x STENCIL: the stencils applied to the Poisson solution */
 } zm_spectral;
 
 /* out = multiplier(k) * in on the reciprocal layout; |k|^2 = (2 pi / length)^2
- * * sum of signed integer wavenumbers squared.  The multiplier is evaluated
- * in FP64 for every precision.  Real multipliers, so the operator is
- * self-adjoint; in == out is allowed. */
+ * * sum of signed integer wavenumbers squared.  The stencil symbols use
+ * theta = 2 pi k / N per axis: Laplacian p0 * sum (2 cos theta - 2), dx4
+ * i (8 sin theta_x - sin 2 theta_x) / (6 p1), theta_x the packed-column axis.
+ * The multiplier is evaluated in FP64 for every precision; in == out is
+ * allowed.  POISSON, HEAT and SCALE are real (self-adjoint); the _adjoint
+ * form applies the complex conjugate multiplier. */
 zm_status zm_spectral_multiply(zm_plan* plan, const void* in, void* out, zm_spectral kind, double p0, double p1,
                                double length, void* stream);
+zm_status zm_spectral_multiply_adjoint(zm_plan* plan, const void* in, void* out, zm_spectral kind, double p0,
+                                       double p1, double length, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* index maps (host, pure, thread-safe): zmodel/utils.h:99-320             */
@@ -202,6 +210,12 @@ void zm_reciprocal_to_global_3d(const zm_shape* s, int rid, int cid, int did, in
                                 int* kr, int* kc);
 void zm_global_to_reciprocal_3d(const zm_shape* s, int kd, int kr, int kc, int* rid, int* cid, int* did, int* ld,
                                 int* lr, int* lc);
+
+/* init_host / init_host_3d (zmodel/utils.cu:9-81) for every rank of the
+ * virtual grid: host buffer of N^d elements of `ncomp` doubles (2 = Complex,
+ * 1 = Scalar) receives the global linear index of each point in the packed
+ * layout (imaginary part 0).  Host only. */
+zm_status zm_init_host(const zm_shape* s, double* host_grid, int ncomp);
 
 /* Whole-field index tables on the device: for every element of a field the
  * global linear index of its real-space point (gr*N+gc / (gd*N+gr)*N+gc) or

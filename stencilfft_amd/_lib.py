@@ -28,7 +28,7 @@ STATUS_NAMES = {
 ZM_FP64, ZM_FP32, ZM_FP32_MFMA = 0, 1, 2
 ZM_FLAG_TWO_STAGE = 1
 ZM_COMPLEX, ZM_REAL = 0, 1
-ZM_SPECTRAL_POISSON, ZM_SPECTRAL_HEAT, ZM_SPECTRAL_SCALE = 0, 1, 2
+ZM_SPECTRAL_POISSON, ZM_SPECTRAL_HEAT, ZM_SPECTRAL_SCALE, ZM_SPECTRAL_STENCIL, ZM_SPECTRAL_POISSON_STENCIL = range(5)
 STAGES = ["line", "stage_a", "stage_b", "repack", "laplace", "deriv", "spectral", "convert"]
 
 
@@ -78,6 +78,7 @@ SIGNATURES = {
     "zm_dx4_adjoint": (_I, [_VP, _VP, _VP, _D, _I, _VP]),
     "zm_dy4_adjoint": (_I, [_VP, _VP, _VP, _D, _I, _VP]),
     "zm_spectral_multiply": (_I, [_VP, _VP, _VP, _I, _D, _D, _D, _VP]),
+    "zm_spectral_multiply_adjoint": (_I, [_VP, _VP, _VP, _I, _D, _D, _D, _VP]),
     "zm_packed_to_block_cyclic": (_I, [_SP, _I]),
     "zm_block_cyclic_to_packed": (_I, [_SP, _I]),
     "zm_real_to_global": (None, [_SP, _I, _I, _I, _I, _IP, _IP]),
@@ -92,6 +93,7 @@ SIGNATURES = {
     "zm_global_to_reciprocal": (None, [_SP, _I, _I, _IP, _IP, _IP, _IP]),
     "zm_reciprocal_to_global_3d": (None, [_SP, _I, _I, _I, _I, _I, _I, _IP, _IP, _IP]),
     "zm_global_to_reciprocal_3d": (None, [_SP, _I, _I, _I, _IP, _IP, _IP, _IP, _IP, _IP]),
+    "zm_init_host": (_I, [_SP, ctypes.POINTER(ctypes.c_double), _I]),
     "zm_real_index_table": (_I, [_VP, _VP, _VP]),
     "zm_reciprocal_index_table": (_I, [_VP, _VP, _VP]),
     "zm_natural_to_packed": (_I, [_VP, _VP, _VP, _VP]),

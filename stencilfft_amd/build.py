@@ -70,9 +70,25 @@ def build(verbose: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB_PATH)
+    build_example()
     if verbose:
         print(f"built {LIB_PATH}")
     return LIB_PATH
+
+
+EXAMPLE_SRC = os.path.join(HERE, "examples", "example.cpp")
+EXAMPLE_BIN = os.path.join(HERE, "examples", "zmodel_example")
+
+
+def build_example() -> str:
+    """The C++ drop-in example, linked against libstencilfft.so (rpath $ORIGIN/..)."""
+    include = os.path.join(os.path.dirname(HERE), "include")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", EXAMPLE_SRC, f"-I{include}", f"-L{HERE}", "-lstencilfft",
+           "-Wl,-rpath,$ORIGIN/..", "-o", EXAMPLE_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{res.stdout}\n{res.stderr}")
+    return EXAMPLE_BIN
 
 
 if __name__ == "__main__":

@@ -70,7 +70,7 @@ hipError_t launch_twiddles(double2* tw64, float2* tw32, int n, hipStream_t strea
 // reciprocal-space multiplier
 template <typename T>
 hipError_t launch_spectral(const Cx<T>* in, Cx<T>* out, const Geo& g, int batch, int kind, double p0, double p1,
-                           double kscale, hipStream_t stream);
+                           double kscale, bool conj, hipStream_t stream);
 
 // index tables and natural <-> packed conversion
 hipError_t launch_index_table(int64_t* out, const Geo& g, bool reciprocal, hipStream_t stream);

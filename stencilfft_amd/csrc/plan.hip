@@ -538,20 +538,33 @@ zm_status zm_dy4_adjoint(zm_plan* p, const void* in, void* out, double delta, zm
     return deriv(p, in, out, delta, element, stream, zm::OP_DY4, true);
 }
 
-zm_status zm_spectral_multiply(zm_plan* p, const void* in, void* out, zm_spectral kind, double p0, double p1,
-                               double length, void* stream) {
+static zm_status spectral(zm_plan* p, const void* in, void* out, zm_spectral kind, double p0, double p1,
+                          double length, void* stream, bool conj) {
     if (!p || !in || !out) return ZM_ERR_INVALID_ARG;
-    if (kind != ZM_SPECTRAL_POISSON && kind != ZM_SPECTRAL_HEAT && kind != ZM_SPECTRAL_SCALE) return ZM_ERR_INVALID_ARG;
+    if ((int)kind < (int)ZM_SPECTRAL_POISSON || (int)kind > (int)ZM_SPECTRAL_POISSON_STENCIL) return ZM_ERR_INVALID_ARG;
     if (!(length > 0.0)) return ZM_ERR_INVALID_ARG;
+    // the dx4 symbol is defined on the 2D layout only, like zm_dx4
+    if ((kind == ZM_SPECTRAL_STENCIL || kind == ZM_SPECTRAL_POISSON_STENCIL) && p1 != 0.0 && p->shape.dims != 2)
+        return ZM_ERR_INVALID_ARG;
     const double kscale = 2.0 * M_PI / length;
     const hipStream_t st = (hipStream_t)stream;
     return launch_status(timed_launch(p, ZM_STAGE_SPECTRAL, st, [&] {
         if (p->precision == ZM_FP64)
             return zm::launch_spectral<double>((const double2*)in, (double2*)out, p->geo, p->batch, (int)kind, p0, p1,
-                                               kscale, st);
+                                               kscale, conj, st);
         return zm::launch_spectral<float>((const float2*)in, (float2*)out, p->geo, p->batch, (int)kind, p0, p1, kscale,
-                                          st);
+                                          conj, st);
     }));
+}
+
+zm_status zm_spectral_multiply(zm_plan* p, const void* in, void* out, zm_spectral kind, double p0, double p1,
+                               double length, void* stream) {
+    return spectral(p, in, out, kind, p0, p1, length, stream, false);
+}
+
+zm_status zm_spectral_multiply_adjoint(zm_plan* p, const void* in, void* out, zm_spectral kind, double p0, double p1,
+                                       double length, void* stream) {
+    return spectral(p, in, out, kind, p0, p1, length, stream, true);
 }
 
 zm_status zm_real_index_table(zm_plan* p, int64_t* out, void* stream) {
@@ -678,6 +691,34 @@ zm_status zm_timing_read(zm_plan* p, zm_stage stage, int* count, double* total_m
 zm_status zm_timing_reset(zm_plan* p) {
     if (!p) return ZM_ERR_INVALID_ARG;
     drop_timing(p);
+    return ZM_OK;
+}
+
+zm_status zm_init_host(const zm_shape* s, double* host_grid, int ncomp) {
+    zm_dims d;
+    if (!host_grid || (ncomp != 1 && ncomp != 2)) return ZM_ERR_INVALID_ARG;
+    ZM_TRY(zm_shape_check(s, &d));
+    const zm::MapShape m = map_shape(s);
+    const int64_t L = d.l, N = s->n;
+    // every packed position holds the global linear index of its point: the
+    // block-cyclic fill of utils.cu:17-32 followed by vector_pack (utils.h:100-108)
+    for (int id = 0; id < d.nranks; id++) {
+        const int cid = id % s->p;
+        const int rid = (id / s->p) % s->p;
+        const int did = s->dims == 3 ? id / (s->p * s->p) : 0;
+        const int64_t ndep = s->dims == 3 ? L : 1;
+        for (int64_t ld = 0; ld < ndep; ld++)
+            for (int64_t lr = 0; lr < L; lr++)
+                for (int64_t lc = 0; lc < L; lc++) {
+                    const int64_t gr = zm::local_row_to_global(m, rid, (int)lr);
+                    const int64_t gc = zm::packed_col_to_global(m, cid, (int)lc);
+                    const int64_t gd = s->dims == 3 ? zm::local_row_to_global(m, did, (int)ld) : 0;
+                    const double v = (double)((gd * N + gr) * N + gc);
+                    const int64_t at = (int64_t)id * d.local + (ld * L + lr) * L + lc;
+                    host_grid[at * ncomp] = v;
+                    if (ncomp == 2) host_grid[at * ncomp + 1] = 0.0;
+                }
+    }
     return ZM_OK;
 }
 

@@ -80,37 +80,76 @@ __device__ __forceinline__ int64_t real_linear(const Geo& g, const MapShape& ms,
     return ((int64_t)gd * g.n + gr) * g.n + gc;
 }
 
+// Symbol of the periodic 5-/7-point Laplacian (scaled by `lap`) plus, in 2D,
+// the 4th-order x-derivative [1, -8, 0, 8, -1] / (12 delta) when delta != 0:
+// for the mode exp(2 pi i k x / N), theta = 2 pi k / N,
+//   Laplacian: lap * sum_d (2 cos theta_d - 2) = -4 lap sum_d sin^2(theta_d / 2)
+//   dx4:       i (8 sin theta_x - sin 2 theta_x) / (6 delta)
+// (the sin^2 form has no cancellation at small k).  kx is the packed-column
+// (gc) wavenumber, the axis zm_dx4 differentiates.
+__device__ __forceinline__ void stencil_symbol(const Geo& g, int s0, int s1, int s2, double lap, double delta,
+                                               double* re, double* im) {
+    const double inv_n = 1.0 / (double)g.n;
+    double acc = 0.0;
+    const int s[3] = {s0, s1, s2};
+    for (int d = 0; d < g.dims; ++d) {
+        const double h = sinpi((double)s[d] * inv_n);  // sin(theta_d / 2)
+        acc += h * h;
+    }
+    *re = -4.0 * lap * acc;
+    *im = 0.0;
+    if (delta != 0.0) {
+        const double t = 2.0 * (double)s0 * inv_n;  // theta_x / pi
+        *im = (8.0 * sinpi(t) - sinpi(2.0 * t)) / (6.0 * delta);
+    }
+}
+
+// kind: 0 Poisson -1/|k|^2 (k = 0 -> 0), 1 heat exp(-p0 |k|^2 p1), 2 constant
+// p0, 3 stencil symbol (Laplacian scale p0, dx4 delta p1), 4 Poisson times the
+// stencil symbol.  conj = 1 applies the complex conjugate multiplier (the
+// adjoint); the multiplier is formed in FP64 for every storage precision.
 template <typename T>
 __global__ void __launch_bounds__(256) spectral_kernel(const Cx<T>* in, Cx<T>* out, Geo g, int kind, double p0,
-                                                       double p1, double kscale) {
+                                                       double p1, double kscale, int conj) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.field) return;
     const MapShape ms = make_map_shape(g.n, g.p, g.b);
     int s0, s1, s2;
     recip_freq(g, ms, i, &s0, &s1, &s2);
     const double k2 = kscale * kscale * ((double)s0 * s0 + (double)s1 * s1 + (double)s2 * s2);
-    double m;
-    if (kind == 0) m = (k2 == 0.0) ? 0.0 : -1.0 / k2;  // Poisson: 1 / (-|k|^2), mean mode removed
-    else if (kind == 1) m = exp(-p0 * k2 * p1);         // heat: exp(-nu |k|^2 dt)
-    else m = p0;
+    const double poisson = (k2 == 0.0) ? 0.0 : -1.0 / k2;  // 1 / (-|k|^2), mean mode removed
+    double mr, mi = 0.0;
+    if (kind == 0) mr = poisson;
+    else if (kind == 1) mr = exp(-p0 * k2 * p1);  // heat: exp(-nu |k|^2 dt)
+    else if (kind == 2) mr = p0;
+    else {
+        stencil_symbol(g, s0, s1, s2, p0, p1, &mr, &mi);
+        if (kind == 4) {
+            mr *= poisson;
+            mi *= poisson;
+        }
+    }
+    if (conj) mi = -mi;
     const int64_t at = (int64_t)blockIdx.y * g.field + i;
     const Cx<T> v = in[at];
-    out[at] = Cx<T>{(T)((double)v.x * m), (T)((double)v.y * m)};
+    const double vx = (double)v.x, vy = (double)v.y;
+    out[at] = Cx<T>{(T)(vx * mr - vy * mi), (T)(vx * mi + vy * mr)};
 }
 
 template <typename T>
 hipError_t launch_spectral(const Cx<T>* in, Cx<T>* out, const Geo& g, int batch, int kind, double p0, double p1,
-                           double kscale, hipStream_t stream) {
+                           double kscale, bool conj, hipStream_t stream) {
     const unsigned threads = 256;
     const dim3 grid((unsigned)((g.field + threads - 1) / threads), (unsigned)batch);
-    hipLaunchKernelGGL((spectral_kernel<T>), grid, dim3(threads), 0, stream, in, out, g, kind, p0, p1, kscale);
+    hipLaunchKernelGGL((spectral_kernel<T>), grid, dim3(threads), 0, stream, in, out, g, kind, p0, p1, kscale,
+                       conj ? 1 : 0);
     return hipGetLastError();
 }
 
 template hipError_t launch_spectral<float>(const float2*, float2*, const Geo&, int, int, double, double, double,
-                                           hipStream_t);
+                                           bool, hipStream_t);
 template hipError_t launch_spectral<double>(const double2*, double2*, const Geo&, int, int, double, double, double,
-                                            hipStream_t);
+                                            bool, hipStream_t);
 
 __global__ void __launch_bounds__(256) index_table_kernel(int64_t* out, Geo g, int reciprocal) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -11,8 +11,10 @@
 // neighbour, including the lane shift the reference applies at the global x
 // boundary (laplace.cu:148, 160; dx.cu:32, 57).
 //
-// Arithmetic follows the reference's evaluation order term by term with FMA
-// contraction off, so the FP64 results are bit-identical to it.
+// FP64 arithmetic follows the reference's evaluation order term by term with
+// FMA contraction off, so the FP64 results are bit-identical to it.  The
+// reference has no FP32 path; FP32 uses a first-difference evaluation order
+// that stays accurate on smooth fields (see lap_combine).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -119,39 +121,66 @@ __device__ __forceinline__ int64_t encode(const Geo& g, int did, int rid, int ci
     return (int64_t)id * g.local + (((int64_t)ld << g.ll) + lr) * g.l + lc;
 }
 
+// Sum of the neighbours minus 2d x centre.  REF: the reference's order
+// (laplace.cu:117-167: ((((0 + above) + below) + left) + right) - 4 c), used
+// for FP64 and bit-identical to it.  Otherwise (FP32) the first-difference
+// form ((above - c) + (below - c)) + ((left - c) + (right - c)): for smooth
+// data the neighbour differences are exact (Sterbenz), so the only rounding
+// falls on the small result instead of cancelling digits of the large terms.
+template <bool REF, typename O, typename E, typename S>
+__device__ __forceinline__ E lap_combine(E a, E b, E l, E r, E f, E k, E c, bool d3) {
+    if (REF) {
+        E s = O::zero();
+        s = O::add(s, a);
+        s = O::add(s, b);
+        s = O::add(s, l);
+        s = O::add(s, r);
+        if (d3) {
+            s = O::add(s, f);
+            s = O::add(s, k);
+            return O::sub(s, O::mul(c, (S)6));
+        }
+        return O::sub(s, O::mul(c, (S)4));
+    }
+    E y = O::add(O::sub(a, c), O::sub(b, c));
+    E x = O::add(O::sub(l, c), O::sub(r, c));
+    E s = O::add(y, x);
+    if (d3) s = O::add(s, O::add(O::sub(f, c), O::sub(k, c)));
+    return s;
+}
+
 template <typename E, typename S>
 __global__ void __launch_bounds__(256) laplace_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, S scale,
                                                       int apply_scale) {
 #pragma clang fp contract(off)
     using O = EOps<E>;
+    constexpr bool REF = sizeof(S) == 8;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.field) return;
     const E* src = in + (int64_t)blockIdx.y * g.field;
     const Pt p = decode(g, (int)i);
     int r2, l2, c2, x2;
-    E s = O::zero();
     // above / below: block-cyclic rows (laplace.cu:100-140)
     bc_walk(g, p.rid, p.lr, -1, &r2, &l2);
-    s = O::add(s, src[encode(g, p.did, r2, p.cid, p.ld, l2, p.lc)]);
+    const E a = src[encode(g, p.did, r2, p.cid, p.ld, l2, p.lc)];
     bc_walk(g, p.rid, p.lr, +1, &r2, &l2);
-    s = O::add(s, src[encode(g, p.did, r2, p.cid, p.ld, l2, p.lc)]);
+    const E b = src[encode(g, p.did, r2, p.cid, p.ld, l2, p.lc)];
     // left / right: packed columns (laplace.cu:142-165)
     const int j = p.lc >> g.lvec, jj = p.lc & (g.vec - 1);
     x_walk(g, p.cid, j, jj, -1, &c2, &x2);
-    s = O::add(s, src[encode(g, p.did, p.rid, c2, p.ld, p.lr, x2)]);
+    const E l = src[encode(g, p.did, p.rid, c2, p.ld, p.lr, x2)];
     x_walk(g, p.cid, j, jj, +1, &c2, &x2);
-    s = O::add(s, src[encode(g, p.did, p.rid, c2, p.ld, p.lr, x2)]);
+    const E r = src[encode(g, p.did, p.rid, c2, p.ld, p.lr, x2)];
     const E c = src[i];
+    E f = c, k = c;
     if (g.dims == 3) {
         // front / back: block-cyclic depth (laplace3d.cu:181-219)
         bc_walk(g, p.did, p.ld, -1, &r2, &l2);
-        s = O::add(s, src[encode(g, r2, p.rid, p.cid, l2, p.lr, p.lc)]);
+        f = src[encode(g, r2, p.rid, p.cid, l2, p.lr, p.lc)];
         bc_walk(g, p.did, p.ld, +1, &r2, &l2);
-        s = O::add(s, src[encode(g, r2, p.rid, p.cid, l2, p.lr, p.lc)]);
-        s = O::sub(s, O::mul(c, (S)6));
-    } else {
-        s = O::sub(s, O::mul(c, (S)4));
+        k = src[encode(g, r2, p.rid, p.cid, l2, p.lr, p.lc)];
     }
+    E s = lap_combine<REF, O, E, S>(a, b, l, r, f, k, c, g.dims == 3);
     if (apply_scale) s = O::mul(s, scale);
     out[(int64_t)blockIdx.y * g.field + i] = s;
 }
@@ -180,11 +209,17 @@ __global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E
             f[k] = src[encode(g, 0, a, p.cid, 0, b2, p.lc)];
         }
     }
-    E s = O::zero();
-    s = O::add(s, O::mul(f[0], (S)-8));
-    s = O::add(s, f[1]);
-    s = O::add(s, O::mul(f[2], (S)8));
-    s = O::sub(s, f[3]);
+    E s;
+    if (sizeof(S) == 8) {
+        s = O::zero();
+        s = O::add(s, O::mul(f[0], (S)-8));
+        s = O::add(s, f[1]);
+        s = O::add(s, O::mul(f[2], (S)8));
+        s = O::sub(s, f[3]);
+    } else {
+        // FP32: 8 (f(+1) - f(-1)) - (f(+2) - f(-2)), differences of nearby values first
+        s = O::sub(O::mul(O::sub(f[2], f[0]), (S)8), O::sub(f[3], f[1]));
+    }
     s = O::div(s, denom);
     if (negate) s = O::mul(s, (S)-1);
     out[(int64_t)blockIdx.y * g.field + i] = s;
@@ -246,12 +281,8 @@ __global__ void __launch_bounds__(256) laplace2d_strip_kernel(const E* __restric
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int pos = tid + 256 * k;
-            E acc = O::zero();
-            acc = O::add(acc, up[k]);
-            acc = O::add(acc, dn[k]);
-            acc = O::add(acc, row[pos]);
-            acc = O::add(acc, row[pos + 2 * VEC]);
-            acc = O::sub(acc, O::mul(ce[k], (S)4));
+            E acc = lap_combine<sizeof(S) == 8, O, E, S>(up[k], dn[k], row[pos], row[pos + 2 * VEC], ce[k], ce[k],
+                                                          ce[k], false);
             if (apply_scale) acc = O::mul(acc, scale);
             dst[oc + pos] = acc;
         }

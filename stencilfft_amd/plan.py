@@ -247,15 +247,20 @@ class Plan:
         L.check(fn(self._h, _ptr(x), _ptr(out), float(delta), self._elem(x), _stream(stream)), "zm_dy4")
         return out
 
+    SPECTRAL_KINDS = {"poisson": L.ZM_SPECTRAL_POISSON, "heat": L.ZM_SPECTRAL_HEAT, "scale": L.ZM_SPECTRAL_SCALE,
+                      "stencil": L.ZM_SPECTRAL_STENCIL, "poisson_stencil": L.ZM_SPECTRAL_POISSON_STENCIL}
+
     def spectral_multiply(self, x, kind: str, p0: float = 0.0, p1: float = 0.0, length: float = 1.0, out=None,
-                          stream=None):
-        kinds = {"poisson": L.ZM_SPECTRAL_POISSON, "heat": L.ZM_SPECTRAL_HEAT, "scale": L.ZM_SPECTRAL_SCALE}
+                          stream=None, adjoint: bool = False):
+        """out = m(k) x on the reciprocal layout.  "stencil" / "poisson_stencil"
+        take p0 = Laplacian scale and p1 = dx4 delta (0: no dx4 term)."""
         if out is None:
             out = self._out(x, None)
         else:
             self._check_tensor(out)
-        L.check(L._lib().zm_spectral_multiply(self._h, _ptr(x), _ptr(out), kinds[kind], float(p0), float(p1),
-                                             float(length), _stream(stream)), "zm_spectral_multiply")
+        fn = L._lib().zm_spectral_multiply_adjoint if adjoint else L._lib().zm_spectral_multiply
+        L.check(fn(self._h, _ptr(x), _ptr(out), self.SPECTRAL_KINDS[kind], float(p0), float(p1), float(length),
+                   _stream(stream)), "zm_spectral_multiply")
         return out
 
     # ---- per-stage timing (zm_timing_*) ----

@@ -117,3 +117,16 @@ def test_product_does_not_import_the_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "liboracle" not in text and "oracle/" not in text, f
+
+
+@pytest.mark.parametrize("dims,n,p,b", [(2, 16, 2, 2), (2, 64, 4, 2), (3, 16, 2, 2), (3, 32, 2, 4)])
+def test_init_host_matches_oracle(oracle_mod, dims, n, p, b):
+    import numpy as np
+    from stencilfft_amd import _lib as L
+    s = oracle_mod.Shape(dims, n, p, b)
+    for ncomp, dt in ((2, np.complex128), (1, np.float64)):
+        got = np.zeros(s.total, dtype=dt)
+        st = L._lib().zm_init_host(ctypes.byref(L.ZmShape(dims, n, p, b)),
+                                   got.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ncomp)
+        assert st == 0
+        assert np.array_equal(got, s.init_host(dt))

@@ -371,3 +371,23 @@ def test_error_paths():
     with pytest.raises(ZmError) as e:
         plan(2, 64, 2, 2, "fp32_mfma")
     assert e.value.status == 3
+
+
+@pytest.mark.parametrize("n,p,b,kmax", [(2048, 4, 8, 16), (4096, 4, 16, 512)])
+def test_fp32_stencils_accurate_on_smooth_fields(n, p, b, kmax):
+    """Smooth fields make sum-of-neighbours minus centre cancel; the FP32
+    stencils must still agree with FP64 on the same rounded input."""
+    from stencilfft_amd.fields import mode_fields
+    note(f"fp32 stencils on smooth fields N={n} kmax={kmax}")
+    p32 = plan(2, n, p, b, "fp32")
+    p64 = plan(2, n, p, b, "fp64")
+    x64 = mode_fields(p64, 64, kmax, seed=41)
+    x32 = x64.to(torch.complex64)
+    x64r = x32.to(torch.complex128)
+
+    def rel(a, r):
+        return (torch.linalg.vector_norm(a.to(torch.complex128) - r) / torch.linalg.vector_norm(r)).item()
+
+    assert rel(p32.laplace(x32, float(n * n)), p64.laplace(x64r, float(n * n))) < 1e-6
+    assert rel(p32.dx4(x32, 1.0 / n), p64.dx4(x64r, 1.0 / n)) < 1e-6
+    assert rel(p32.dy4(x32, 1.0 / n), p64.dy4(x64r, 1.0 / n)) < 1e-6
