@@ -44,10 +44,11 @@ def rel_errors(a, ref, torch):
 
 
 def run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream):
+    dims = plan.shape.dims
     for _ in range(steps):
         plan.laplace(x, scale=float(n * n), out=lap, stream=stream)
         plan.forward(lap, out=y, scratch=scratch, stream=stream)
-        plan.inverse(y, 1.0 / (n * n), out=z, scratch=scratch, stream=stream)
+        plan.inverse(y, 1.0 / float(n) ** dims, out=z, scratch=scratch, stream=stream)
 
 
 def time_path(plan, x, n, steps, warmup, torch, label):
@@ -78,17 +79,21 @@ def time_path(plan, x, n, steps, warmup, torch, label):
     return ms, stages, (lap, y, z)
 
 
-def cpu_baseline(n, p, b, seed):
+def cpu_baseline(dims, n, p, b, seed):
     """FP64 CPU oracle (C port of the reference path, radix-2 DFT mode, OpenMP
-    over the host cores) on one full step of the same workload."""
+    over the host cores) on one full step of the same shape, and beside it a
+    stand-in for the reference's heFFTe+FFTW CPU path (absent here): scipy.fft
+    (pocketfft) forward + inverse on the natural layout with one worker per
+    core, plus a numpy periodic Laplacian."""
     import numpy as np
 
     import oracle
     oracle.build()
     threads = len(os.sched_getaffinity(0))
-    s = oracle.Shape(2, n, p, b)
+    s = oracle.Shape(dims, n, p, b)
     rng = np.random.default_rng(seed)
     x = (rng.uniform(-1, 1, s.total) + 1j * rng.uniform(-1, 1, s.total)).astype(np.complex128)
+    field = float(n) ** dims
     oracle.set_dft_mode(1)
     try:
         t0 = time.perf_counter()
@@ -96,22 +101,45 @@ def cpu_baseline(n, p, b, seed):
         t1 = time.perf_counter()
         y = s.forward_fft(lap)
         t2 = time.perf_counter()
-        s.inverse_fft(y, scale=1.0 / (n * n))
+        s.inverse_fft(y, scale=1.0 / field)
         t3 = time.perf_counter()
     finally:
         oracle.set_dft_mode(0)
-    flops = 2 * 5.0 * n * n * math.log2(n * n)
+    del lap, y
+    flops = 2 * 5.0 * field * math.log2(field)
     log(f"cpu oracle ({threads} threads): laplace {t1 - t0:.2f} s, forward {t2 - t1:.2f} s, inverse {t3 - t2:.2f} s")
+    standin = None
+    try:
+        import scipy.fft as sfft
+        xn = x.reshape((n,) * dims)
+        del x
+        u0 = time.perf_counter()
+        lapn = -2.0 * dims * xn
+        for ax in range(dims):
+            lapn += np.roll(xn, 1, ax) + np.roll(xn, -1, ax)
+        u1 = time.perf_counter()
+        yn = sfft.fftn(lapn, workers=threads)
+        u2 = time.perf_counter()
+        sfft.ifftn(yn, workers=threads)
+        u3 = time.perf_counter()
+        log(f"cpu scipy stand-in ({threads} workers): laplace {u1 - u0:.2f} s, fftn {u2 - u1:.2f} s, "
+            f"ifftn {u3 - u2:.2f} s")
+        standin = {"value": flops / (u3 - u0) / 1e9, "unit": "GFLOP/s", "cores": threads,
+                   "sample": "numpy periodic Laplacian + scipy.fft fftn/ifftn (pocketfft) of the same complex128 "
+                             f"field on the natural layout, {u3 - u0:.2f} s, forward alone {u2 - u1:.2f} s"}
+    except ImportError:
+        pass
     return {
         "value": flops / (t3 - t0) / 1e9,
         "unit": "GFLOP/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"one full step (5-point Laplacian, forward FFT, inverse FFT) of a 2D N={n} P={p} B={b} "
-                   f"complex128 field in the FP64 C oracle (radix-2 DFT mode, OpenMP); "
-                   f"{t3 - t0:.2f} s, forward alone {t2 - t1:.2f} s"),
+        "sample": (f"one full step (Laplacian, forward FFT, inverse FFT) of a {dims}D N={n} P={p} B={b} complex128 "
+                   f"U[-1,1) field in the FP64 C oracle (radix-2 DFT mode, OpenMP); {t3 - t0:.2f} s, forward "
+                   f"alone {t2 - t1:.2f} s"),
         "seconds_per_step": t3 - t0,
         "forward_seconds": t2 - t1,
+        "standin_heffte_fftw": standin,
     }
 
 
@@ -129,9 +157,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_mfma"])
-    ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--p", type=int, default=4)
-    ap.add_argument("--b", type=int, default=32)
+    ap.add_argument("--dims", type=int, default=2, choices=[2, 3])
+    ap.add_argument("--n", type=int, default=None, help="default 8192 (2D) / 512 (3D)")
+    ap.add_argument("--p", type=int, default=None, help="default 4 (2D) / 2 (3D)")
+    ap.add_argument("--b", type=int, default=None, help="default 32 (2D) / 16 (3D)")
     ap.add_argument("--two-stage", action="store_true", help="force the two-kernel-per-direction path")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -154,7 +183,11 @@ def main():
         print(json.dumps({"error": "no GPU"}))
         sys.exit(1)
 
-    n, p, b = args.n, args.p, args.b
+    dims = args.dims
+    dflt = {2: (8192, 4, 32), 3: (512, 2, 16)}[dims]
+    n = args.n or dflt[0]
+    p = args.p or dflt[1]
+    b = args.b or dflt[2]
     if args.copy_calibration:
         src = torch.ones(n * n, dtype=torch.complex64, device="cuda")
         dst = torch.empty_like(src)
@@ -175,29 +208,35 @@ def main():
                           "bytes_per_copy_write": src.numel() * src.element_size(), "ms_per_copy": round(ms, 4),
                           "frac_of_peak": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}))
         return
-    log(f"workload 2D N={n} P={p} B={b} precision={args.precision} steps={args.steps} warmup={args.warmup}")
-    from stencilfft_amd.fields import mode_field_natural
-    nat = mode_field_natural(n, 2, 256, n // 8, args.seed, noise=1e-3)
-    plan64 = Plan(2, n, p, b, precision="fp64", two_stage=args.two_stage)
-    x64 = plan64.natural_to_packed(nat)
-    del nat
+    log(f"workload {dims}D N={n} P={p} B={b} precision={args.precision} steps={args.steps} warmup={args.warmup}")
+    plan64 = Plan(dims, n, p, b, precision="fp64", two_stage=args.two_stage)
+    if dims == 2:
+        from stencilfft_amd.fields import mode_field_natural
+        nat = mode_field_natural(n, 2, 256, n // 8, args.seed, noise=1e-3)
+        x64 = plan64.natural_to_packed(nat)
+        del nat
+    else:
+        from stencilfft_amd.fields import uniform_fields
+        x64 = uniform_fields(plan64, args.seed)
     if args.precision == "fp64":
         plan = plan64
         x = x64
     else:
-        plan = Plan(2, n, p, b, precision=args.precision, two_stage=args.two_stage)
+        plan = Plan(dims, n, p, b, precision=args.precision, two_stage=args.two_stage)
         x = x64.to(torch.complex64)
     torch.cuda.synchronize()
 
     ms, stages, outs = time_path(plan, x, n, args.steps, args.warmup, torch, args.precision)
     esize = x.element_size()
-    field = n * n
+    field = n ** dims
     flops = 2 * 5.0 * field * math.log2(field)
     gflops = flops / (ms * 1e-3) / 1e9
-    # bytes per step: Section 8(d) accounting (2 sweeps per direction, fused
-    # pack/exchange/twiddle/repack) and the textbook bound (1 sweep per direction)
-    bytes_s8d = (2 * (2 * 2 * 2 * field * esize)) + 2 * field * esize
-    bytes_textbook = (2 * (2 * 2 * field * esize)) + 2 * field * esize
+    # bytes per step: 2 sweeps per direction and FFT (pack, exchange, twiddle
+    # and repack fused into adjacent loads and stores) and the textbook bound
+    # (1 sweep per direction), plus one sweep for the stencil
+    sweep = 2 * field * esize
+    bytes_s8d = 2 * (2 * dims) * sweep + sweep
+    bytes_textbook = 2 * dims * sweep + sweep
     step_frac = bytes_s8d / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
 
     # dominant kernel and its roofline (textbook bytes: one read + one write of
@@ -254,7 +293,7 @@ def main():
     if not args.no_cpu_baseline:
         del outs
         torch.cuda.empty_cache()
-        cpu = cpu_baseline(n, p, b, args.seed)
+        cpu = cpu_baseline(dims, n, p, b, args.seed)
 
     line = {
         "metric": "FFT GFLOP/s (5N·log2N), % HBM BW peak; FP32 vs FP64 path at stated error",
@@ -270,11 +309,13 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"2D N={n} P={p} B={b} ({p * p} virtual ranks): 5-point Laplacian (h=1/N) -> forward FFT "
-                         f"-> inverse FFT (1/N^2), complex {'128' if args.precision == 'fp64' else '64'}"),
+            "workload": (f"{dims}D N={n} P={p} B={b} ({p ** dims} virtual ranks): {5 if dims == 2 else 7}-point "
+                         f"Laplacian (h=1/N) -> forward FFT -> inverse FFT (1/N^{dims}), complex "
+                         f"{'128' if args.precision == 'fp64' else '64'}"),
             "n": n, "p": p, "b": b, "precision": args.precision,
             "path": "two-stage" if args.two_stage else "fused",
-            "field": f"256 modes |k|<=N/8, amp N(0,1)/(1+|k|^2), + 1e-3 U[-1,1) noise, seed {args.seed}",
+            "field": (f"256 modes |k|<=N/8, amp N(0,1)/(1+|k|^2), + 1e-3 U[-1,1) noise, seed {args.seed}"
+                      if dims == 2 else f"re, im U[-1,1) (torch CUDA generator), seed {args.seed}"),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

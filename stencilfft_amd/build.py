@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libstencilfft.so")
-SOURCES = ["plan.hip", "fft.hip", "line_fast.hip", "stencil.hip", "spectral.hip", "mfma.hip"]
+SOURCES = ["plan.hip", "fft.hip", "line_fast.hip", "stencil.hip", "spectral.hip"]
 ARCH = "gfx950"
 
 # No packed-FP32 instructions: hipcc would otherwise form v_pk_* FP32 ops from
@@ -61,7 +61,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = True) -> str:
     """Compile every HIP source for gfx950 and link libstencilfft.so."""
     os.makedirs(BUILD_DIR, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = list(SOURCES)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB_PATH + ".tmp"

@@ -70,10 +70,13 @@ __device__ __forceinline__ C tw_n(const C* ta, const C* tb, int e) {
 //           (the M-point DFTs); lanes run along k1.
 // PRE: multiply element i of transform bt by W_N^(i * kk(bt)) first.
 // TOGLOBAL: instead of writing LDS, hand each output to `sink`.
+// OOP: read `src`, write `dst` (a different LDS buffer) as soon as each
+// transform is done, one barrier at the end; otherwise src == dst and all
+// reads finish (barrier) before the writes start.
 template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool TOGLOBAL, bool FULLTAB, typename Sink>
-__device__ __forceinline__ void fast_pass(Cx<T>* lds, const Cx<T>* ts, int ls, const Cx<T>* ta, const Cx<T>* tb,
-                                          int tid, Sink&& sink) {
+          bool TOGLOBAL, bool FULLTAB, bool OOP, typename Sink>
+__device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, int ls, const Cx<T>* ta,
+                                          const Cx<T>* tb, int tid, Sink&& sink) {
     using C = Cx<T>;
     constexpr int n = 1 << LN_T;
     constexpr int nbf = n / R;
@@ -104,7 +107,7 @@ __device__ __forceinline__ void fast_pass(Cx<T>* lds, const Cx<T>* ts, int ls, c
             kk = k1;
         }
 #pragma unroll
-        for (int r = 0; r < R; r++) v[k][r] = lds[pidx(base + (j + r * nbf) * ES)];
+        for (int r = 0; r < R; r++) v[k][r] = src[pidx(base + (j + r * nbf) * ES)];
         if (PRE) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -125,13 +128,19 @@ __device__ __forceinline__ void fast_pass(Cx<T>* lds, const Cx<T>* ts, int ls, c
         const int jm = j & (NS - 1);
         const int oidx = (j - jm) * R + jm;  // output index within the transform, before + r*NS
         obase[k] = TOGLOBAL ? oidx : base + oidx * ES;
+        if constexpr (OOP && !TOGLOBAL) {
+#pragma unroll
+            for (int r = 0; r < R; r++) dst[pidx(obase[k] + r * NS * ES)] = v[k][r];
+        }
     }
-    if (!TOGLOBAL) {
+    if constexpr (OOP && !TOGLOBAL) {
+        __syncthreads();
+    } else if constexpr (!TOGLOBAL) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < BPT; k++) {
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[pidx(obase[k] + r * NS * ES)] = v[k][r];
+            for (int r = 0; r < R; r++) dst[pidx(obase[k] + r * NS * ES)] = v[k][r];
         }
         __syncthreads();
     } else {
@@ -144,29 +153,195 @@ __device__ __forceinline__ void fast_pass(Cx<T>* lds, const Cx<T>* ts, int ls, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-precision MFMA radix-16 pass (ZM_FP32_MFMA).
+//
+// A radix-16 DFT of 16 transforms is one real GEMM Y = W X with K = 32
+// (k = 2r + c: input r, c = re/im) and 32 output rows (2s + c'), done as two
+// 16x16 output tiles of v_mfma_f32_16x16x32_f16.  Operand layouts of that
+// instruction (wave64): lane l holds A[l & 15][8 (l >> 4) + t], B[8 (l >> 4) + t]
+// [l & 15] and D[4 (l >> 4) + t][l & 15].  So lane l works on transform
+// column l & 15 with inputs r = 4g..4g+3 and outputs s = 2g, 2g+1 (tile 0) and
+// 8+2g, 9+2g (tile 1), g = l >> 4.
+// Data are split x = hi + lo into two FP16 values (round to nearest), W the
+// same way, and all four products are accumulated in FP32, so the only error
+// is the ~2^-23 relative representation error of the split.  FP16 range: the
+// kernel scales each line group by a power of two when it stages it into LDS
+// (largest |re|, |im| of the group in [2^6, 2^7), MFMA_SCALE_LOG2) and undoes
+// it exactly on the final store.  A radix-R pass grows magnitudes by at most
+// R, and at most 16 * 8 of growth precedes the last MFMA pass of a direction,
+// so every FP16 operand stays below 2^14; values 2^-30 below the group's
+// largest still keep their absolute error under 2^-31 of it.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// A operands of one lane: hi and lo halves of W for output tiles 0 and 1,
+// kept in LDS (MFMA_W_BYTES) rather than in VGPRs.
+struct Mfma16W {
+    half8_t hi[2], lo[2];
+};
+constexpr int MFMA_W_BYTES = 64 * (int)sizeof(Mfma16W);
+constexpr int MFMA_SCALE_LOG2 = 7;
+constexpr int MFMA_MAX_BYTES = 2 * 16 * 4;  // two slots of per-wave maxima (<= 16 waves)
+
+template <bool FWD>
+__device__ __forceinline__ Mfma16W mfma16_weights(int lane) {
+    Mfma16W w;
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int srow = 8 * t + (i >> 1), cp = i & 1;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int r = 4 * g + (q >> 1), c = q & 1;
+            double sn, cs;
+            sincospi((double)((srow * r) & 15) / 8.0, &sn, &cs);  // W_16^(s r) = cs - i sn (forward)
+            const double wi = FWD ? -sn : sn;
+            // real form of the complex product: [[re, -im], [im, re]]
+            const double a = (cp == 0) ? (c == 0 ? cs : -wi) : (c == 0 ? wi : cs);
+            const _Float16 h = (_Float16)a;
+            w.hi[t][q] = h;
+            w.lo[t][q] = (_Float16)(a - (double)h);
+        }
+    }
+    return w;
+}
+
+template <typename T, int LN_T, int NS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
+          bool TOGLOBAL, bool FULLTAB, bool OOP, typename Sink>
+__device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, int ls, const Cx<T>* ta,
+                                            const Cx<T>* tb, int tid, const Mfma16W* wtab, Sink&& sink) {
+    static_assert(sizeof(T) == 4, "split-precision MFMA pass is FP32 only");
+    static_assert(OOP || TOGLOBAL, "the MFMA pass writes each group as soon as it is done");
+    using C = Cx<T>;
+    constexpr int R = 16;
+    constexpr int n = 1 << LN_T;
+    constexpr int nbf = n / R;
+    constexpr int LNBF = fast::Log2<nbf>::value;
+    constexpr int VEC = 1 << LVEC;
+    constexpr int N = 1 << LN;
+    constexpr int ES = CONTIG ? 1 : VEC;
+    static_assert(NT % 64 == 0 && (NT * EPT) == (1 << LE), "work split");
+    const int lane = tid & 63, wave = tid >> 6, col = lane & 15, g = lane >> 4;
+    // Register budget (the fused kernel runs at up to 256 VGPRs): the groups
+    // are not unrolled and the A operands of a tile are read from LDS right
+    // before its four MFMAs.
+    const Mfma16W* wl = wtab + lane;
+#pragma unroll 1
+    for (int q = 0; q < 4; q++) {
+        const int u = wave * 64 + q * 16 + col;  // transform handled by this column
+        int base, j, kk;
+        if (CONTIG) {
+            j = u & (nbf - 1);
+            const int bt = u >> LNBF;
+            base = bt << LN_T;
+            kk = bt & ((N >> LN_T) - 1);
+        } else {
+            const int k1 = u & (VEC - 1);
+            const int rest = u >> LVEC;
+            j = rest & (nbf - 1);
+            const int line = rest >> LNBF;
+            base = (line << LN) + k1;
+            kk = k1;
+        }
+        C x[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = 4 * g + i;
+            x[i] = src[pidx(base + (j + r * nbf) * ES)];
+            if (PRE) {
+                const int e = ((j + r * nbf) * kk) & (N - 1);
+                if constexpr (FULLTAB) x[i] = ctw<FWD>(x[i], ta[e]);
+                else x[i] = ctw<FWD>(x[i], cmul(ta[e & 63], tb[e >> 6]));
+            }
+            if (NS > 1) {
+                const int jm = j & (NS - 1);
+                const int step = (1 << ls) / (NS * R);
+                if (r > 0) x[i] = ctw<FWD>(x[i], ts[jm * r * step]);
+            }
+        }
+        half8_t bh, bl;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const _Float16 hr = (_Float16)x[i].x, hi_ = (_Float16)x[i].y;
+            bh[2 * i] = hr;
+            bh[2 * i + 1] = hi_;
+            bl[2 * i] = (_Float16)(x[i].x - (float)hr);
+            bl[2 * i + 1] = (_Float16)(x[i].y - (float)hi_);
+        }
+        const int jm = j & (NS - 1);
+        const int oidx = (j - jm) * R + jm;
+        const int ob = TOGLOBAL ? oidx : base + oidx * ES;
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const half8_t whi = wl->hi[t], wlo = wl->lo[t];
+            floatx4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(wlo, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(wlo, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(whi, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(whi, bh, acc, 0, 0, 0);
+            // this lane's outputs of tile t: s = 8t + 2g and 8t + 2g + 1
+            const int s0 = 8 * t + 2 * g;
+            const C y0{acc[0], acc[1]}, y1{acc[2], acc[3]};
+            if constexpr (TOGLOBAL) {
+                sink(u, ob + s0 * NS, y0);
+                sink(u, ob + (s0 + 1) * NS, y1);
+            } else {
+                dst[pidx(ob + s0 * NS * ES)] = y0;
+                dst[pidx(ob + (s0 + 1) * NS * ES)] = y1;
+            }
+        }
+    }
+    if constexpr (!TOGLOBAL) __syncthreads();
+}
+
 struct NoSink {
     template <typename C>
     __device__ void operator()(int, int, C) const {}
 };
 
 // Run passes PASS..NPASS-1 of a length-2^LN_T transform (all in LDS).
+// Run passes PASS..NPASS-1 of a length-2^LN_T transform, all in LDS.  With
+// OOP the passes alternate between the buffers `cur` and `alt`; the return
+// value is the buffer that holds the result.
 template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool FULLTAB>
-__device__ __forceinline__ void fast_passes_lds(Cx<T>* lds, const Cx<T>* ts, int ls, const Cx<T>* ta,
-                                                const Cx<T>* tb, int tid) {
+          bool FULLTAB, bool MF, bool OOP>
+__device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const Cx<T>* ts, int ls, const Cx<T>* ta,
+                                                  const Cx<T>* tb, int tid, const Mfma16W* w) {
     if constexpr (PASS < NPASS) {
         constexpr int R = fast::Radix<LN_T, PASS>::value;
         constexpr int NS = 1 << (4 * PASS);
-        fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB>(lds, ts, ls, ta, tb,
-                                                                                                    tid, NoSink{});
-        fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB>(lds, ts, ls, ta, tb,
-                                                                                                tid);
+        Cx<T>* dst = OOP ? alt : cur;
+        if constexpr (MF && R == 16)
+            mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
+                cur, dst, ts, ls, ta, tb, tid, w, NoSink{});
+        else
+            fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
+                cur, dst, ts, ls, ta, tb, tid, NoSink{});
+        return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP>(
+            dst, OOP ? cur : alt, ts, ls, ta, tb, tid, w);
+    } else {
+        return cur;
     }
+}
+
+// last pass of a direction, straight to HBM through `sink`
+template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
+          bool FULLTAB, bool MF, typename Sink>
+__device__ __forceinline__ void last_pass(const Cx<T>* cur, const Cx<T>* ts, int ls, const Cx<T>* ta,
+                                          const Cx<T>* tb, int tid, const Mfma16W* w, Sink&& sink) {
+    if constexpr (MF && R == 16)
+        mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false>(cur, nullptr, ts, ls, ta,
+                                                                                           tb, tid, w, sink);
+    else
+        fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false>(cur, nullptr, ts, ls, ta,
+                                                                                            tb, tid, sink);
 }
 
 // Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
 // iteration.  a.lpb is the number of iterations (line groups) per workgroup.
-template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB>
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF>
 __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     line_fast_kernel(const Cx<T>* in, Cx<T>* out, Geo g, LineArgs a, const Cx<T>* __restrict__ tables, T scale,
                      int apply_scale, int groups) {
@@ -179,8 +354,11 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     constexpr int NPV = fast::NumPasses<LVEC>::value, NPM = fast::NumPasses<LM>::value;
     constexpr int NTWO = S + 64 + (N / 64 > 0 ? N / 64 : 1);  // W_S + two-level W_N
     extern __shared__ unsigned char smem_raw[];
+    // MF: two LDS data buffers, the radix passes ping-pong between them
+    constexpr bool OOP = MF;
     C* lds = reinterpret_cast<C*>(smem_raw);
-    C* tabs = lds + padded_elems(E);
+    C* lds2 = OOP ? lds + padded_elems(E) : lds;
+    C* tabs = lds + padded_elems(E) * (OOP ? 2 : 1);
     const int tid = threadIdx.x;
 
     // global table layout: [W_S | A (64) | B (N/64) | W_N (N)]
@@ -193,6 +371,15 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     const C* ts = tabs;
     const C* ta = tabs + S;
     const C* tb = tabs + S + 64;
+    // MF: the per-lane MFMA weights and two slots of per-wave maxima follow
+    // the tables in LDS
+    Mfma16W* w = reinterpret_cast<Mfma16W*>(tabs + (FULLTAB ? S + N : NTWO));
+    float* wmax = reinterpret_cast<float*>(w + 64);
+    constexpr int NW = NT / 64;
+    static_assert(!MF || NW <= 16, "maxima slots");
+    if constexpr (MF) {
+        if (tid < 64) w[tid] = mfma16_weights<FWD>(tid);
+    }
 
     const int L = g.l, ll = g.ll, lp = g.lp;
     const int lvp = g.lvec - g.lp;
@@ -224,6 +411,19 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             pre[k] = src0[row_addr(o, gm, q0 + l, pos)];
         }
     };
+    // MF: largest |re|, |im| of the prefetched group, reduced over the wave
+    // and left in maxima slot `slot` for the next stage()
+    auto group_max = [&](int slot) {
+        if constexpr (MF) {
+            float m = 0.0f;
+#pragma unroll
+            for (int k = 0; k < EPT; k++) m = fmaxf(m, fmaxf(fabsf(pre[k].x), fabsf(pre[k].y)));
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+            if ((tid & 63) == 0) wmax[slot * 16 + (tid >> 6)] = m;
+        }
+    };
+    T s0 = 1, inv_s0 = 1;
     auto stage = [&]() {
 #pragma unroll
         for (int k = 0; k < EPT; k++) {
@@ -234,14 +434,28 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 const int gm = rem >> ll, pos = rem & (L - 1);
                 li = (l << LN) + ((pos >> lvp) << LVEC) + (gm << lvp) + (pos & (VP - 1));
             }
-            lds[pidx(li)] = pre[k];
+            if constexpr (MF) lds[pidx(li)] = C{pre[k].x * s0, pre[k].y * s0};
+            else lds[pidx(li)] = pre[k];
         }
     };
 
     int grp = blockIdx.x;
-    if (grp < groups) fetch(grp);
-    __syncthreads();  // tables
-    for (; grp < groups; grp += gridDim.x) {
+    int it = 0;
+    if (grp < groups) {
+        fetch(grp);
+        if constexpr (MF) group_max(0);
+    }
+    __syncthreads();  // tables, maxima
+    for (; grp < groups; grp += gridDim.x, ++it) {
+        if constexpr (MF) {
+            float m = 0.0f;
+#pragma unroll
+            for (int v = 0; v < NW; v++) m = fmaxf(m, wmax[(it & 1) * 16 + v]);
+            int ex;
+            (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+            s0 = ldexpf(1.0f, MFMA_SCALE_LOG2 - ex);
+            inv_s0 = ldexpf(1.0f, ex - MFMA_SCALE_LOG2);
+        }
         stage();
         __syncthreads();
         const int nxt = grp + gridDim.x;
@@ -253,44 +467,51 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
 
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
-            fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB>(lds, ts, LS, ta, tb, tid);
+            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP>(
+                lds, lds2, ts, LS, ta, tb, tid, w);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
-            fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB>(lds, ts, LS, ta, tb, tid);
+            cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP>(
+                cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
             constexpr int PL = NPM - 1;
             constexpr int RL = fast::Radix<LM, PL>::value;
             constexpr int NSL = 1 << (4 * PL);
-            fast_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, true, FULLTAB>(
-                lds, ts, LS, ta, tb, tid, [&](int u, int k2, C v) {
+            last_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, FULLTAB, MF>(
+                cur, ts, LS, ta, tb, tid, w, [&](int u, int k2, C v) {
                     // transform (line, k1): k1 = u & (VEC-1), line = u >> (LVEC + log2(nbf))
                     constexpr int nbf = (1 << LM) / RL;
                     const int k1 = u & (VEC - 1);
                     const int line = (u >> LVEC) >> fast::Log2<nbf>::value;
                     const int gq = k1 >> lvp, fr = k1 & (VP - 1);
+                    if constexpr (MF) v = C{v.x * inv_s0, v.y * inv_s0};
                     dst0[row_addr(o, gq, q0 + line, fr + (k2 << lvp))] = v;
                 });
         } else {
             // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
             // the conjugate inter-stage twiddle; the last pass stores to HBM
-            fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB>(lds, ts, LS, ta, tb, tid);
-            fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB>(lds, ts, LS, ta, tb,
-                                                                                                tid);
+            C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP>(
+                lds, lds2, ts, LS, ta, tb, tid, w);
+            cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP>(
+                cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
             constexpr int PL = NPV - 1;
             constexpr int RL = fast::Radix<LVEC, PL>::value;
             constexpr int NSL = 1 << (4 * PL);
-            const T sc = scale;
-            fast_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, true, FULLTAB>(
-                lds, ts, LS, ta, tb, tid, [&](int u, int jj, C v) {
+            const T sc = MF ? scale * inv_s0 : scale;
+            last_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, FULLTAB, MF>(
+                cur, ts, LS, ta, tb, tid, w, [&](int u, int jj, C v) {
                     // transform (line, r): r = bt & (M-1), line = bt >> LM
                     constexpr int nbf = (1 << LVEC) / RL;
                     const int bt = u >> fast::Log2<nbf>::value;
                     const int r = bt & (M - 1), line = bt >> LM;
                     const int gm = r >> g.lb, j = r & (g.b - 1);
-                    if (apply_scale) v = C{v.x * sc, v.y * sc};
+                    if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
                     dst0[row_addr(o, gm, q0 + line, (j << LVEC) + jj)] = v;
                 });
         }
-        __syncthreads();  // the next stage() overwrites LDS
+        if constexpr (MF) {
+            if (nxt < groups) group_max((it + 1) & 1);
+        }
+        __syncthreads();  // the next stage() overwrites LDS; maxima
     }
 }
 
@@ -307,49 +528,49 @@ int lines_per_iter_log2(int n) {
 }
 
 // LDS bytes of the specialised kernel for element size `es`
-size_t fast_lds(size_t es, int vec, int m, bool fulltab) {
+size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false) {
     const int n = vec * m, s = vec > m ? vec : m;
     const int e = n << lines_per_iter_log2(n);
     const int tab = fulltab ? s + n : s + 64 + (n / 64 > 0 ? n / 64 : 1);
-    return es * (padded_elems(e) + tab);
+    return es * (padded_elems(e) * (mf ? 2 : 1) + tab) + (mf ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0);
 }
 
 }  // namespace
 
 template <typename T>
-bool fast_line_fulltab(int vec, int m) {
-    return fast_lds(sizeof(Cx<T>), vec, m, true) <= kLdsLimit;
+bool fast_line_fulltab(int vec, int m, bool mf = false) {
+    return fast_lds(sizeof(Cx<T>), vec, m, true, mf) <= kLdsLimit;
 }
 
-template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB>
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF>
 static const void* fast_fn() {
-    return (const void*)&line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB>;
+    return (const void*)&line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB, MF>;
 }
 
-template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB>
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF>
 static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a,
                                    const Cx<T>* tables, int batch, double scale, int grid_x, hipStream_t stream) {
     constexpr int LN = LVEC + LM, N = 1 << LN, E = N << LLPI;
-    const size_t lds = fast_lds(sizeof(Cx<T>), 1 << LVEC, 1 << LM, FULLTAB);
-    const void* fn = fast_fn<T, LVEC, LM, LLPI, FWD, FULLTAB>();
+    const size_t lds = fast_lds(sizeof(Cx<T>), 1 << LVEC, 1 << LM, FULLTAB, MF);
+    const void* fn = fast_fn<T, LVEC, LM, LLPI, FWD, FULLTAB, MF>();
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int groups = (int)(g.field / N) >> LLPI;
     const dim3 grid((unsigned)(grid_x < groups ? grid_x : groups), (unsigned)batch);
-    hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB>), grid, dim3(E / EPT), lds, stream, in, out,
+    hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB, MF>), grid, dim3(E / EPT), lds, stream, in, out,
                        g, a, tables, (T)scale, scale != 1.0 ? 1 : 0, groups);
     return hipGetLastError();
 }
 
 // Dispatch over the instantiated (VEC, M) pairs; `op` is called with the
 // kernel's template arguments.
-template <typename T, typename Op>
+template <typename T, bool MF, typename Op>
 static auto with_fast(int vec, int m, bool fwd, Op&& op) {
 #define ZM_LF1(V, MM, F, FT)                                                          \
-    if (vec == (1 << V) && m == (1 << MM) && fwd == F && fast_line_fulltab<T>(vec, m) == FT) \
-        return op.template run<V, MM, ((V + MM) >= 12 ? 0 : 12 - (V + MM)), F, FT>();
+    if (vec == (1 << V) && m == (1 << MM) && fwd == F && fast_line_fulltab<T>(vec, m, MF) == FT) \
+        return op.template run<V, MM, ((V + MM) >= 12 ? 0 : 12 - (V + MM)), F, FT, MF>();
 #define ZM_LF(V, MM) ZM_LF1(V, MM, true, true) ZM_LF1(V, MM, true, false) ZM_LF1(V, MM, false, true) ZM_LF1(V, MM, false, false)
     ZM_LF(4, 4) ZM_LF(4, 5) ZM_LF(4, 6) ZM_LF(4, 7)
     ZM_LF(5, 4) ZM_LF(5, 5) ZM_LF(5, 6) ZM_LF(5, 7)
@@ -371,9 +592,9 @@ struct LaunchOp {
     double scale;
     int grid_x;
     hipStream_t stream;
-    template <int V, int MM, int LL, bool F, bool FT>
+    template <int V, int MM, int LL, bool F, bool FT, bool MF>
     hipError_t run() {
-        return launch_fast_inst<T, V, MM, LL, F, FT>(in, out, g, a, tables, batch, scale, grid_x, stream);
+        return launch_fast_inst<T, V, MM, LL, F, FT, MF>(in, out, g, a, tables, batch, scale, grid_x, stream);
     }
     hipError_t none() { return hipErrorInvalidValue; }
 };
@@ -381,10 +602,10 @@ struct LaunchOp {
 template <typename T>
 struct OccOp {
     int vec, m;
-    template <int V, int MM, int LL, bool F, bool FT>
+    template <int V, int MM, int LL, bool F, bool FT, bool MF>
     int run() {
-        const void* fn = fast_fn<T, V, MM, LL, F, FT>();
-        const size_t lds = fast_lds(sizeof(Cx<T>), vec, m, FT);
+        const void* fn = fast_fn<T, V, MM, LL, F, FT, MF>();
+        const size_t lds = fast_lds(sizeof(Cx<T>), vec, m, FT, MF);
         const int nthreads = ((1 << (V + MM)) << LL) / EPT;
         if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int blocks = 0;
@@ -398,18 +619,28 @@ template <typename T>
 bool fast_line_supported(int vec, int m) {
     const bool okv = vec == 16 || vec == 32 || vec == 64 || vec == 128;
     const bool okm = m == 16 || m == 32 || m == 64 || m == 128;
-    return okv && okm && (int64_t)vec * m <= 8192 && fast_lds(sizeof(Cx<T>), vec, m, false) <= kLdsLimit;
+    return okv && okm && (int64_t)vec * m <= 8192 &&
+           fast_lds(sizeof(Cx<T>), vec, m, false, sizeof(T) == 4) <= kLdsLimit;
 }
 
 template <typename T>
-int fast_line_occupancy(int vec, int m, bool fwd) {
-    return with_fast<T>(vec, m, fwd, OccOp<T>{vec, m});
+int fast_line_occupancy(int vec, int m, bool fwd, bool mf) {
+    if constexpr (sizeof(T) == 4) {
+        if (mf) return with_fast<T, true>(vec, m, fwd, OccOp<T>{vec, m});
+    }
+    if (mf) return 0;
+    return with_fast<T, false>(vec, m, fwd, OccOp<T>{vec, m});
 }
 
 template <typename T>
 hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
-                            int batch, bool fwd, double scale, int grid_x, hipStream_t stream) {
-    return with_fast<T>(g.vec, g.m, fwd, LaunchOp<T>{in, out, g, a, tables, batch, scale, grid_x, stream});
+                            int batch, bool fwd, double scale, int grid_x, bool mf, hipStream_t stream) {
+    const LaunchOp<T> op{in, out, g, a, tables, batch, scale, grid_x, stream};
+    if constexpr (sizeof(T) == 4) {
+        if (mf) return with_fast<T, true>(g.vec, g.m, fwd, LaunchOp<T>(op));
+    }
+    if (mf) return hipErrorInvalidValue;  // the split-precision passes are FP32 only
+    return with_fast<T, false>(g.vec, g.m, fwd, LaunchOp<T>(op));
 }
 
 // FP64 twiddle tables of a line (N = vec*m, S = max(vec, m)), rounded for
@@ -453,11 +684,11 @@ hipError_t launch_line_tables(double2* t64, float2* t32, int vec, int m, hipStre
 
 template bool fast_line_supported<float>(int, int);
 template bool fast_line_supported<double>(int, int);
-template int fast_line_occupancy<float>(int, int, bool);
-template int fast_line_occupancy<double>(int, int, bool);
+template int fast_line_occupancy<float>(int, int, bool, bool);
+template int fast_line_occupancy<double>(int, int, bool, bool);
 template hipError_t launch_line_fast<float>(const float2*, float2*, const Geo&, const LineArgs&, const float2*, int,
-                                            bool, double, int, hipStream_t);
+                                            bool, double, int, bool, hipStream_t);
 template hipError_t launch_line_fast<double>(const double2*, double2*, const Geo&, const LineArgs&, const double2*,
-                                             int, bool, double, int, hipStream_t);
+                                             int, bool, double, int, bool, hipStream_t);
 
 }  // namespace zm

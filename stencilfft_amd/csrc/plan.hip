@@ -144,7 +144,11 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
     if (batch < 1 || batch > 65535) return ZM_ERR_INVALID_ARG;
     zm_dims d;
     ZM_TRY(zm_shape_check(shape, &d));
-    if (precision == ZM_FP32_MFMA) return ZM_ERR_UNSUPPORTED;
+    const bool mfma = precision == ZM_FP32_MFMA;
+    // the split-precision radix-16 passes live in the specialised fused line
+    // pass only
+    if (mfma && ((flags & ZM_FLAG_TWO_STAGE) || !zm::fast_line_supported<float>(d.vec, d.m)))
+        return ZM_ERR_UNSUPPORTED;
 
     const int n = shape->n;
     const int64_t rows = (int64_t)d.local / d.l;  // L^(dims-1) local rows per rank
@@ -223,10 +227,10 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
             e = zm::launch_line_tables(p->ltab64, p->ltab32, d.vec, d.m, nullptr);
             int cus = 0;
             if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-            const int of = fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, true)
-                                : zm::fast_line_occupancy<float>(d.vec, d.m, true);
-            const int oi = fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, false)
-                                : zm::fast_line_occupancy<float>(d.vec, d.m, false);
+            const int of = fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, true, false)
+                                : zm::fast_line_occupancy<float>(d.vec, d.m, true, mfma);
+            const int oi = fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, false, false)
+                                : zm::fast_line_occupancy<float>(d.vec, d.m, false, mfma);
             if (e == hipSuccess && of > 0 && oi > 0) {
                 p->fast = true;
                 p->fast_grid_fwd = of * cus;
@@ -235,13 +239,13 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
         }
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
+    if (e != hipSuccess || (mfma && !p->fast)) {
         (void)hipFree(p->tw64);
         (void)hipFree(p->tw32);
         if (p->ltab64) (void)hipFree(p->ltab64);
         if (p->ltab32) (void)hipFree(p->ltab32);
         delete p;
-        return ZM_ERR_DEVICE;
+        return e != hipSuccess ? ZM_ERR_DEVICE : ZM_ERR_UNSUPPORTED;
     }
     *plan = p;
     return ZM_OK;
@@ -325,7 +329,8 @@ struct Exec {
         if (p->fast && compute) {
             return launch_status(timed_launch(p, ZM_STAGE_LINE, st, [&] {
                 return zm::launch_line_fast<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, a, line_tables(),
-                                               p->batch, fwd, scale, fwd ? p->fast_grid_fwd : p->fast_grid_inv, st);
+                                               p->batch, fwd, scale, fwd ? p->fast_grid_fwd : p->fast_grid_inv,
+                                               p->precision == ZM_FP32_MFMA, st);
             }));
         }
         return launch_status(timed_launch(p, ZM_STAGE_LINE, st, [&] {

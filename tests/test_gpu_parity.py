@@ -352,6 +352,58 @@ def test_poisson_solve_recovers_laplacian_inverse(oracle_mod):
 
 # ---------------------------------------------------------------------------
 # error paths
+# shapes with radix-16 passes in both the VEC- and the M-point DFTs, where the
+# split-precision MFMA variant replaces them (VEC, M in {16, ..., 128})
+MFMA_SHAPES = [(2, 256, 2, 8), (2, 512, 2, 16), (2, 1024, 4, 8), (2, 2048, 2, 16), (2, 4096, 4, 32),
+               (2, 2048, 4, 4), (3, 256, 2, 8), (3, 256, 4, 4)]
+
+
+@pytest.mark.parametrize("dims,n,p,b", MFMA_SHAPES)
+def test_mfma_line_pass_vs_oracle(oracle_mod, dims, n, p, b):
+    """FP32 data with the radix-16 DFT stages as split-precision (hi/lo FP16,
+    FP32-accumulated) MFMA GEMMs: same bar as the FP32 vector-ALU path."""
+    note(f"fp32_mfma {dims}D N={n} P={p} B={b}")
+    s = oracle_mod.Shape(dims, n, p, b)
+    rng = np.random.default_rng(3 * n + b)
+    x = (rng.uniform(-1, 1, s.total) + 1j * rng.uniform(-1, 1, s.total)).astype(np.complex64).astype(np.complex128)
+    pl = plan(dims, n, p, b, "fp32_mfma")
+    oracle_mod.set_dft_mode(1)
+    try:
+        want = s.forward_fft(x)
+        scale = 1.0 / n ** dims
+        w_in = want.astype(np.complex64).astype(np.complex128)
+        zwant = s.inverse_fft(w_in, scale=scale)
+    finally:
+        oracle_mod.set_dft_mode(0)
+    y = host(pl.forward(dev(x, torch.complex64))).astype(np.complex128)
+    z = host(pl.inverse(dev(w_in, torch.complex64), scale)).astype(np.complex128)
+    note(f"  forward rel L2 {rel_l2(y, want):.2e} max {rel_max(y, want):.2e}; "
+         f"inverse rel L2 {rel_l2(z, zwant):.2e} max {rel_max(z, zwant):.2e}")
+    assert rel_l2(y, want) < 1e-6 and rel_max(y, want) < 1e-5
+    assert rel_l2(z, zwant) < 1e-6 and rel_max(z, zwant) < 1e-5
+
+
+def test_mfma_spiky_spectrum():
+    """Per-transform power-of-two scaling: a field of a few huge modes plus
+    tiny noise (a spectrum spanning ~12 decades) keeps the FP32 error bar."""
+    from stencilfft_amd.fields import mode_fields
+    note("fp32_mfma spiky spectrum")
+    n, p, b = 2048, 2, 16
+    p64 = plan(2, n, p, b, "fp64")
+    pm = plan(2, n, p, b, "fp32_mfma")
+    x64 = mode_fields(p64, 4, 8, seed=5) * 1e6 + mode_fields(p64, 16, n // 4, seed=6) * 1e-6
+    x32 = x64.to(torch.complex64)
+    x64r = x32.to(torch.complex128)
+    y64 = p64.forward(x64r)
+    ym = pm.forward(x32).to(torch.complex128)
+    rel = (torch.linalg.vector_norm(ym - y64) / torch.linalg.vector_norm(y64)).item()
+    z64 = p64.inverse(y64.to(torch.complex64).to(torch.complex128), 1.0 / n ** 2)
+    zm = pm.inverse(y64.to(torch.complex64), 1.0 / n ** 2).to(torch.complex128)
+    relz = (torch.linalg.vector_norm(zm - z64) / torch.linalg.vector_norm(z64)).item()
+    note(f"  forward rel L2 {rel:.2e}, inverse {relz:.2e}")
+    assert rel < 1e-6 and relz < 1e-6
+
+
 # ---------------------------------------------------------------------------
 
 def test_error_paths():
@@ -369,7 +421,10 @@ def test_error_paths():
         p1.dx4(torch.zeros(64 * 64, dtype=torch.float64, device="cuda"))
     assert e.value.status == 3
     with pytest.raises(ZmError) as e:
-        plan(2, 64, 2, 2, "fp32_mfma")
+        plan(2, 64, 2, 2, "fp32_mfma")  # M = 4: no specialised line pass
+    assert e.value.status == 3
+    with pytest.raises(ZmError) as e:
+        plan(2, 1024, 2, 16, "fp32_mfma", two_stage=True)
     assert e.value.status == 3
 
 
