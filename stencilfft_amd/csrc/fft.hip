@@ -272,6 +272,108 @@ __global__ void __launch_bounds__(256) repack_kernel(const Cx<T>* in, Cx<T>* out
     }
 }
 
+// The same permutation with compile-time tiles of TX = 2^LTX values of x by
+// TY = 2^LTY values of y, 256 threads, and 16-byte global accesses (two FP32
+// or one FP64 complex value): every index is a shift, every thread keeps
+// TX*TY/256 values in flight, and each global row segment is TY (x/y side) or
+// TX (packed side) contiguous elements.  The LDS pitch TY + 1 puts the
+// column reads of the packed side on distinct banks.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 2;
+    __device__ static void split(const float4& v, float2* c) {
+        c[0] = float2{v.x, v.y};
+        c[1] = float2{v.z, v.w};
+    }
+    __device__ static float4 join(const float2* c) { return float4{c[0].x, c[0].y, c[1].x, c[1].y}; }
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 1;
+    __device__ static void split(const double2& v, double2* c) { c[0] = v; }
+    __device__ static double2 join(const double2* c) { return c[0]; }
+};
+
+template <typename T, bool FWD, int LTX, int LTY>
+__global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restrict__ in, Cx<T>* __restrict__ out,
+                                                           Geo g, RepackArgs a, int lecount) {
+    using C = Cx<T>;
+    using V16 = Vec16<T>;
+    using Vt = typename V16::type;
+    constexpr int TX = 1 << LTX, TY = 1 << LTY, V = V16::n;
+    constexpr int XA = TY / V, XR = 256 / XA;  // x/y side: accesses per row, rows per sweep
+    constexpr int PA = TX / V, PR = 256 / PA;  // packed side
+    static_assert(XR >= 1 && TX % XR == 0 && PR >= 1 && TY % PR == 0, "tile split");
+    __shared__ C tile[TX][TY + 1];
+    const int tid = threadIdx.x;
+    unsigned rest = blockIdx.x;
+    const int lyb = g.ll - LTY, ltb = g.lvec - LTX;
+    const int yb = (int)(rest & ((1u << lyb) - 1));
+    rest >>= lyb;
+    const int tb = (int)(rest & ((1u << ltb) - 1));
+    rest >>= ltb;
+    const int rho = (int)(rest & (unsigned)(g.b - 1));
+    rest >>= g.lb;
+    const int e = (int)(rest & ((1u << lecount) - 1));
+    const int id = (int)(rest >> lecount);
+    const C* src = in + (int64_t)blockIdx.y * g.field + (int64_t)id * g.local;
+    C* dst = out + (int64_t)blockIdx.y * g.field + (int64_t)id * g.local;
+    const int t0 = tb << LTX, y0 = yb << LTY;
+    // x/y side: row t holds x = rho + B*(t0 + t), columns y0 .. y0 + TY - 1
+    const int xc = (tid % XA) * V, xr = tid / XA;
+    const C* xy = (FWD ? src : dst) + (int64_t)e * a.ei + y0 + xc;
+    // packed side: row y0 + y holds pack(x) = rho*VEC + t0 .. + TX - 1
+    const int pc = (tid % PA) * V, pr = tid / PA;
+    const C* pk = (FWD ? dst : src) + (int64_t)e * a.eo + (int64_t)y0 * a.syo + (rho << g.lvec) + t0 + pc;
+    const int64_t xstep = (int64_t)g.b * a.sxi;
+    const C* xyr = xy + (int64_t)(rho + g.b * (t0 + xr)) * a.sxi;
+    if constexpr (FWD) {
+        Vt r[TX / XR];
+#pragma unroll
+        for (int i = 0; i < TX / XR; i++) r[i] = *reinterpret_cast<const Vt*>(xyr + (int64_t)i * XR * xstep);
+#pragma unroll
+        for (int i = 0; i < TX / XR; i++) {
+            C c[V];
+            V16::split(r[i], c);
+#pragma unroll
+            for (int v = 0; v < V; v++) tile[xr + i * XR][xc + v] = c[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TY / PR; i++) {
+            const int y = pr + i * PR;
+            C c[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) c[v] = tile[pc + v][y];
+            *reinterpret_cast<Vt*>(const_cast<C*>(pk) + (int64_t)y * a.syo) = V16::join(c);
+        }
+    } else {
+        Vt r[TY / PR];
+#pragma unroll
+        for (int i = 0; i < TY / PR; i++) r[i] = *reinterpret_cast<const Vt*>(pk + (int64_t)(pr + i * PR) * a.syo);
+#pragma unroll
+        for (int i = 0; i < TY / PR; i++) {
+            C c[V];
+            V16::split(r[i], c);
+#pragma unroll
+            for (int v = 0; v < V; v++) tile[pc + v][pr + i * PR] = c[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TX / XR; i++) {
+            const int t = xr + i * XR;
+            C c[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) c[v] = tile[t][xc + v];
+            *reinterpret_cast<Vt*>(const_cast<C*>(xyr) + (int64_t)i * XR * xstep) = V16::join(c);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -355,6 +457,23 @@ hipError_t launch_stage_b(const Cx<T>* in, Cx<T>* out, const Geo& g, const Stage
 template <typename T>
 hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const RepackArgs& a, int batch, bool fwd,
                          hipStream_t stream) {
+    // tiled path: VEC >= 16 and L >= 32 (x tile 32 or 16, y tile 64 or 32)
+    if (g.vec >= 16 && g.l >= 32) {
+        const int ltx = g.vec >= 32 ? 5 : 4, lty = g.l >= 64 ? 6 : 5;
+        int lecount = 0;
+        while ((1 << lecount) < a.ecount) lecount++;
+        const unsigned blocks =
+            (unsigned)((int64_t)g.nranks * a.ecount * g.b * (g.vec >> ltx) * (g.l >> lty));
+        const dim3 grid(blocks, (unsigned)batch);
+#define ZM_REPACK(F, X, Y)                                                                                     \
+    if (fwd == F && ltx == X && lty == Y) {                                                                    \
+        hipLaunchKernelGGL((repack_tiled_kernel<T, F, X, Y>), grid, dim3(256), 0, stream, in, out, g, a, lecount); \
+        return hipGetLastError();                                                                              \
+    }
+        ZM_REPACK(true, 5, 6) ZM_REPACK(true, 5, 5) ZM_REPACK(true, 4, 6) ZM_REPACK(true, 4, 5)
+        ZM_REPACK(false, 5, 6) ZM_REPACK(false, 5, 5) ZM_REPACK(false, 4, 6) ZM_REPACK(false, 4, 5)
+#undef ZM_REPACK
+    }
     const unsigned blocks = (unsigned)(g.nranks * a.ecount * g.b * (g.vec / a.tx) * (g.l / a.ty));
     const dim3 grid(blocks, (unsigned)batch);
     if (fwd) hipLaunchKernelGGL((repack_kernel<T, true>), grid, dim3(256), 0, stream, in, out, g, a);

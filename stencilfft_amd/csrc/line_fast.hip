@@ -176,11 +176,21 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
-// A operands of one lane: hi and lo halves of W for output tiles 0 and 1,
-// kept in LDS (MFMA_W_BYTES) rather than in VGPRs.
+// A operands of one lane for output tile 0 (rows s = 0..7), hi and lo
+// halves, kept in LDS (MFMA_W_BYTES).  Tile 1 needs no storage:
+// W_16^((s+8) r) = (-1)^r W_16^(s r), and this lane's K slots 2..3 and 6..7
+// hold the odd r (r = 4g + q/2), so tile 1's operand is tile 0's with the
+// sign bits of its 2nd and 4th dwords flipped.
 struct Mfma16W {
-    half8_t hi[2], lo[2];
+    half8_t hi, lo;
 };
+typedef unsigned int uint4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ half8_t negate_odd_r(half8_t a) {
+    uint4_t u = __builtin_bit_cast(uint4_t, a);
+    u[1] ^= 0x80008000u;
+    u[3] ^= 0x80008000u;
+    return __builtin_bit_cast(half8_t, u);
+}
 constexpr int MFMA_W_BYTES = 64 * (int)sizeof(Mfma16W);
 constexpr int MFMA_SCALE_LOG2 = 7;
 constexpr int MFMA_MAX_BYTES = 2 * 16 * 4;  // two slots of per-wave maxima (<= 16 waves)
@@ -189,9 +199,8 @@ template <bool FWD>
 __device__ __forceinline__ Mfma16W mfma16_weights(int lane) {
     Mfma16W w;
     const int i = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-        const int srow = 8 * t + (i >> 1), cp = i & 1;
+    {
+        const int srow = i >> 1, cp = i & 1;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int r = 4 * g + (q >> 1), c = q & 1;
@@ -201,8 +210,8 @@ __device__ __forceinline__ Mfma16W mfma16_weights(int lane) {
             // real form of the complex product: [[re, -im], [im, re]]
             const double a = (cp == 0) ? (c == 0 ? cs : -wi) : (c == 0 ? wi : cs);
             const _Float16 h = (_Float16)a;
-            w.hi[t][q] = h;
-            w.lo[t][q] = (_Float16)(a - (double)h);
+            w.hi[q] = h;
+            w.lo[q] = (_Float16)(a - (double)h);
         }
     }
     return w;
@@ -225,8 +234,7 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
     static_assert(NT % 64 == 0 && (NT * EPT) == (1 << LE), "work split");
     const int lane = tid & 63, wave = tid >> 6, col = lane & 15, g = lane >> 4;
     // Register budget (the fused kernel runs at up to 256 VGPRs): the groups
-    // are not unrolled and the A operands of a tile are read from LDS right
-    // before its four MFMAs.
+    // are not unrolled, and each group reads its A operands from LDS.
     const Mfma16W* wl = wtab + lane;
 #pragma unroll 1
     for (int q = 0; q < 4; q++) {
@@ -273,9 +281,11 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
         const int jm = j & (NS - 1);
         const int oidx = (j - jm) * R + jm;
         const int ob = TOGLOBAL ? oidx : base + oidx * ES;
+        const half8_t whi0 = wl->hi, wlo0 = wl->lo;
 #pragma unroll
         for (int t = 0; t < 2; t++) {
-            const half8_t whi = wl->hi[t], wlo = wl->lo[t];
+            const half8_t whi = t ? negate_odd_r(whi0) : whi0;
+            const half8_t wlo = t ? negate_odd_r(wlo0) : wlo0;
             floatx4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
             acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(wlo, bl, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(wlo, bh, acc, 0, 0, 0);
