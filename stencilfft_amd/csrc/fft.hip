@@ -308,6 +308,15 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
     constexpr int XA = TY / V, XR = 256 / XA;  // x/y side: accesses per row, rows per sweep
     constexpr int PA = TX / V, PR = 256 / PA;  // packed side
     static_assert(XR >= 1 && TX % XR == 0 && PR >= 1 && TY % PR == 0, "tile split");
+    // non-temporal for FP64, plain for FP32 (whichever measured faster)
+    auto ldv = [](const Vt* p) -> Vt {
+        if constexpr (sizeof(T) == 8) return ld_nt(p);
+        else return *p;
+    };
+    auto stv = [](Vt* p, Vt v) {
+        if constexpr (sizeof(T) == 8) st_nt(p, v);
+        else *p = v;
+    };
     __shared__ C tile[TX][TY + 1];
     const int tid = threadIdx.x;
     unsigned rest = blockIdx.x;
@@ -334,7 +343,7 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
     if constexpr (FWD) {
         Vt r[TX / XR];
 #pragma unroll
-        for (int i = 0; i < TX / XR; i++) r[i] = *reinterpret_cast<const Vt*>(xyr + (int64_t)i * XR * xstep);
+        for (int i = 0; i < TX / XR; i++) r[i] = ldv(reinterpret_cast<const Vt*>(xyr + (int64_t)i * XR * xstep));
 #pragma unroll
         for (int i = 0; i < TX / XR; i++) {
             C c[V];
@@ -349,12 +358,12 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
             C c[V];
 #pragma unroll
             for (int v = 0; v < V; v++) c[v] = tile[pc + v][y];
-            *reinterpret_cast<Vt*>(const_cast<C*>(pk) + (int64_t)y * a.syo) = V16::join(c);
+            stv(reinterpret_cast<Vt*>(const_cast<C*>(pk) + (int64_t)y * a.syo), V16::join(c));
         }
     } else {
         Vt r[TY / PR];
 #pragma unroll
-        for (int i = 0; i < TY / PR; i++) r[i] = *reinterpret_cast<const Vt*>(pk + (int64_t)(pr + i * PR) * a.syo);
+        for (int i = 0; i < TY / PR; i++) r[i] = ldv(reinterpret_cast<const Vt*>(pk + (int64_t)(pr + i * PR) * a.syo));
 #pragma unroll
         for (int i = 0; i < TY / PR; i++) {
             C c[V];
@@ -369,7 +378,7 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
             C c[V];
 #pragma unroll
             for (int v = 0; v < V; v++) c[v] = tile[t][xc + v];
-            *reinterpret_cast<Vt*>(const_cast<C*>(xyr) + (int64_t)i * XR * xstep) = V16::join(c);
+            stv(reinterpret_cast<Vt*>(const_cast<C*>(xyr) + (int64_t)i * XR * xstep), V16::join(c));
         }
     }
 }

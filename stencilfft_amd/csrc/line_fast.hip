@@ -366,6 +366,10 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     extern __shared__ unsigned char smem_raw[];
     // MF: two LDS data buffers, the radix passes ping-pong between them
     constexpr bool OOP = MF;
+    // VEC = 16: the last pass would store 16 contiguous outputs per thread
+    // (inverse) or VEC/P-element runs (forward), i.e. partial cache lines per
+    // store instruction; instead it stays in LDS and a sweep writes whole rows
+    constexpr bool STAGED = LVEC == 4;
     C* lds = reinterpret_cast<C*>(smem_raw);
     C* lds2 = OOP ? lds + padded_elems(E) : lds;
     C* tabs = lds + padded_elems(E) * (OOP ? 2 : 1);
@@ -407,6 +411,18 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         return (int64_t)rank * g.local + ((int64_t)q << ll) + pos;
     };
 
+    // Global accesses: non-temporal for FP64 (16-byte elements), plain for
+    // FP32, whichever measured faster for each.
+    constexpr bool NTH = sizeof(T) == 8;
+    auto ldg = [](const C* p) -> C {
+        if constexpr (NTH) return ld_nt(p);
+        else return *p;
+    };
+    auto stg = [](C* p, C v) {
+        if constexpr (NTH) st_nt(p, v);
+        else *p = v;
+    };
+
     // register prefetch of one line group: element e = tid + k*NT
     C pre[EPT];
     auto fetch = [&](int grp) {
@@ -418,7 +434,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             const int e = tid + k * NT;
             const int l = e >> LN, rem = e & (N - 1);
             const int gm = rem >> ll, pos = rem & (L - 1);
-            pre[k] = src0[row_addr(o, gm, q0 + l, pos)];
+            pre[k] = ldg(src0 + row_addr(o, gm, q0 + l, pos));
         }
     };
     // MF: largest |re|, |im| of the prefetched group, reduced over the wave
@@ -481,42 +497,75 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 lds, lds2, ts, LS, ta, tb, tid, w);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
-            cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP>(
-                cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
-            constexpr int PL = NPM - 1;
-            constexpr int RL = fast::Radix<LM, PL>::value;
-            constexpr int NSL = 1 << (4 * PL);
-            last_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, FULLTAB, MF>(
-                cur, ts, LS, ta, tb, tid, w, [&](int u, int k2, C v) {
-                    // transform (line, k1): k1 = u & (VEC-1), line = u >> (LVEC + log2(nbf))
-                    constexpr int nbf = (1 << LM) / RL;
-                    const int k1 = u & (VEC - 1);
-                    const int line = (u >> LVEC) >> fast::Log2<nbf>::value;
-                    const int gq = k1 >> lvp, fr = k1 & (VP - 1);
+            if constexpr (STAGED) {
+                cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                // element k2 of transform (line, k1) is at line*N + k1 + k2*VEC;
+                // sweep each destination row (rank gq, position fr + k2*VP)
+#pragma unroll
+                for (int k = 0; k < EPT; k++) {
+                    const int e = tid + k * NT;
+                    const int l = e >> LN, rem = e & (N - 1);
+                    const int gq = rem >> ll, pos = rem & (L - 1);
+                    const int k1 = (gq << lvp) + (pos & (VP - 1)), k2 = pos >> lvp;
+                    C v = cur[pidx((l << LN) + k1 + (k2 << LVEC))];
                     if constexpr (MF) v = C{v.x * inv_s0, v.y * inv_s0};
-                    dst0[row_addr(o, gq, q0 + line, fr + (k2 << lvp))] = v;
-                });
+                    stg(dst0 + row_addr(o, gq, q0 + l, pos), v);
+                }
+            } else {
+                cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                constexpr int PL = NPM - 1;
+                constexpr int RL = fast::Radix<LM, PL>::value;
+                constexpr int NSL = 1 << (4 * PL);
+                last_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, FULLTAB, MF>(
+                    cur, ts, LS, ta, tb, tid, w, [&](int u, int k2, C v) {
+                        // transform (line, k1): k1 = u & (VEC-1), line = u >> (LVEC + log2(nbf))
+                        constexpr int nbf = (1 << LM) / RL;
+                        const int k1 = u & (VEC - 1);
+                        const int line = (u >> LVEC) >> fast::Log2<nbf>::value;
+                        const int gq = k1 >> lvp, fr = k1 & (VP - 1);
+                        if constexpr (MF) v = C{v.x * inv_s0, v.y * inv_s0};
+                        stg(dst0 + row_addr(o, gq, q0 + line, fr + (k2 << lvp)), v);
+                    });
+            }
         } else {
             // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
             // the conjugate inter-stage twiddle; the last pass stores to HBM
             C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP>(
                 lds, lds2, ts, LS, ta, tb, tid, w);
-            cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP>(
-                cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
-            constexpr int PL = NPV - 1;
-            constexpr int RL = fast::Radix<LVEC, PL>::value;
-            constexpr int NSL = 1 << (4 * PL);
             const T sc = MF ? scale * inv_s0 : scale;
-            last_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, FULLTAB, MF>(
-                cur, ts, LS, ta, tb, tid, w, [&](int u, int jj, C v) {
-                    // transform (line, r): r = bt & (M-1), line = bt >> LM
-                    constexpr int nbf = (1 << LVEC) / RL;
-                    const int bt = u >> fast::Log2<nbf>::value;
-                    const int r = bt & (M - 1), line = bt >> LM;
-                    const int gm = r >> g.lb, j = r & (g.b - 1);
+            if constexpr (STAGED) {
+                cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                // element jj of transform (line, r = gm*B + j) is at
+                // line*N + r*VEC + jj = line*N + gm*L + (j*VEC + jj): the
+                // group's LDS order is the destination rows' order
+#pragma unroll
+                for (int k = 0; k < EPT; k++) {
+                    const int e = tid + k * NT;
+                    const int l = e >> LN, rem = e & (N - 1);
+                    C v = cur[pidx(e)];
                     if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
-                    dst0[row_addr(o, gm, q0 + line, (j << LVEC) + jj)] = v;
-                });
+                    stg(dst0 + row_addr(o, rem >> ll, q0 + l, rem & (L - 1)), v);
+                }
+            } else {
+                cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                constexpr int PL = NPV - 1;
+                constexpr int RL = fast::Radix<LVEC, PL>::value;
+                constexpr int NSL = 1 << (4 * PL);
+                last_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, FULLTAB, MF>(
+                    cur, ts, LS, ta, tb, tid, w, [&](int u, int jj, C v) {
+                        // transform (line, r): r = bt & (M-1), line = bt >> LM
+                        constexpr int nbf = (1 << LVEC) / RL;
+                        const int bt = u >> fast::Log2<nbf>::value;
+                        const int r = bt & (M - 1), line = bt >> LM;
+                        const int gm = r >> g.lb, j = r & (g.b - 1);
+                        if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
+                        stg(dst0 + row_addr(o, gm, q0 + line, (j << LVEC) + jj), v);
+                    });
+            }
         }
         if constexpr (MF) {
             if (nxt < groups) group_max((it + 1) & 1);

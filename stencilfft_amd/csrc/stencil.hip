@@ -258,15 +258,15 @@ __global__ void __launch_bounds__(256) laplace2d_strip_kernel(const E* __restric
         const int64_t ou = row_off(g0 - 1, cid), oc = row_off(g0, cid);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            up[k] = src[ou + tid + 256 * k];
-            ce[k] = src[oc + tid + 256 * k];
+            up[k] = ld_nt(src + ou + tid + 256 * k);
+            ce[k] = ld_nt(src + oc + tid + 256 * k);
         }
     }
     for (int t = 0; t < rows_per_strip; t++) {
         const int gr = g0 + t;
         const int64_t od = row_off(gr + 1, cid);
 #pragma unroll
-        for (int k = 0; k < K; k++) dn[k] = src[od + tid + 256 * k];
+        for (int k = 0; k < K; k++) dn[k] = ld_nt(src + od + tid + 256 * k);
         // stage the current row and its halos
 #pragma unroll
         for (int k = 0; k < K; k++) row[VEC + tid + 256 * k] = ce[k];
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) laplace2d_strip_kernel(const E* __restric
             E acc = lap_combine<sizeof(S) == 8, O, E, S>(up[k], dn[k], row[pos], row[pos + 2 * VEC], ce[k], ce[k],
                                                           ce[k], false);
             if (apply_scale) acc = O::mul(acc, scale);
-            dst[oc + pos] = acc;
+            st_nt(dst + oc + pos, acc);
         }
         __syncthreads();
 #pragma unroll
