@@ -287,7 +287,18 @@ def main():
         result_extra["fp64_ms_per_step"] = round(ms64, 4)
         result_extra["speedup_vs_fp64_gpu"] = round(ms64 / ms, 3)
         log(f"errors vs FP64 GPU path: {json.dumps(errs)}")
-        del x64r, outs64
+        del outs64
+        # the FP64 path with the reference's stage structure: per direction
+        # FFT1a + pack + exchange, then unpack + twiddle + FFT1b, materialising
+        # the post-exchange buffer (zmodel/transforms/fft.cu:199-339)
+        if not args.two_stage:
+            plan64s = Plan(dims, n, p, b, precision="fp64", two_stage=True)
+            ms64s, _, outs64s = time_path(plan64s, x64r, n, max(3, min(args.steps, 10)), 2, torch,
+                                          "fp64 two-stage")
+            result_extra["fp64_two_stage_ms_per_step"] = round(ms64s, 4)
+            result_extra["speedup_vs_fp64_two_stage_gpu"] = round(ms64s / ms, 3)
+            del outs64s, plan64s
+        del x64r
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -92,6 +92,19 @@ __device__ __forceinline__ void st_nt(float4* p, float4 v) {
 }
 
 // ---------------------------------------------------------------------------
+// XCD-aware block order.  gfx950 dispatches consecutive workgroups round-robin
+// over its 8 XCDs, each with its own L2.  For stencils, whose neighbour rows
+// sit in the neighbouring blocks, remap so that XCD k runs the k-th contiguous
+// eighth of the grid: neighbours then share an L2.  Identity unless the block
+// count is a multiple of 8.
+// ---------------------------------------------------------------------------
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    if (nb % kXcds) return b;
+    return (b % kXcds) * (nb / kXcds) + b / kXcds;
+}
+
+// ---------------------------------------------------------------------------
 // Register DFTs of radix 2..16 (unnormalised, natural order), written as a
 // decimation-in-time recursion with exact constants for the R-th roots of
 // unity, R <= 16.  FWD uses exp(-2 pi i / R), the inverse its conjugate.

@@ -404,12 +404,20 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     const C* src0 = in + (int64_t)blockIdx.y * g.field;
     C* dst0 = out + (int64_t)blockIdx.y * g.field;
 
-    // global offset (in elements, within the field) of element `pos` of member
-    // `gm`'s row q of group o
-    auto row_addr = [&](int o, int gm, int q, int pos) -> int64_t {
-        const int rank = ((o >> lsg) << (lsg + lp)) + (o & (a.sg - 1)) + (gm << lsg);
-        return (int64_t)rank * g.local + ((int64_t)q << ll) + pos;
+    // Line group grp = exchange group o, rows q0 .. q0 + LPI - 1 of each of its
+    // P member ranks.  Member gm of group o is rank rank0(o) + gm*sg, so
+    // element `pos` of member gm's row q0 + l sits at
+    //   group_base(grp) + (gm << mshift) + (l << ll) + pos,
+    // a uniform 64-bit base plus a 32-bit offset (fields hold < 2^31 elements).
+    const int mshift = lsg + g.dims * ll;  // log2(sg * L^dims)
+    auto group_base = [&](int grp) -> int64_t {
+        const int64_t lam0 = (int64_t)grp << LLPI;
+        const int o = (int)(lam0 / lpg);
+        const int q0 = (int)(lam0 - (int64_t)o * lpg);
+        const int rank0 = ((o >> lsg) << (lsg + lp)) + (o & (a.sg - 1));
+        return (int64_t)rank0 * g.local + ((int64_t)q0 << ll);
     };
+    auto off = [&](int gm, int l, int pos) -> int { return (gm << mshift) + (l << ll) + pos; };
 
     // Global accesses: non-temporal for FP64 (16-byte elements), plain for
     // FP32, whichever measured faster for each.
@@ -426,15 +434,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     // register prefetch of one line group: element e = tid + k*NT
     C pre[EPT];
     auto fetch = [&](int grp) {
-        const int64_t lam0 = (int64_t)grp << LLPI;
-        const int o = (int)(lam0 / lpg);
-        const int q0 = (int)(lam0 - (int64_t)o * lpg);
+        const C* gb = src0 + group_base(grp);
 #pragma unroll
         for (int k = 0; k < EPT; k++) {
             const int e = tid + k * NT;
             const int l = e >> LN, rem = e & (N - 1);
-            const int gm = rem >> ll, pos = rem & (L - 1);
-            pre[k] = ldg(src0 + row_addr(o, gm, q0 + l, pos));
+            pre[k] = ldg(gb + off(rem >> ll, l, rem & (L - 1)));
         }
     };
     // MF: largest |re|, |im| of the prefetched group, reduced over the wave
@@ -487,9 +492,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         const int nxt = grp + gridDim.x;
         if (nxt < groups) fetch(nxt);
 
-        const int64_t lam0 = (int64_t)grp << LLPI;
-        const int o = (int)(lam0 / lpg);
-        const int q0 = (int)(lam0 - (int64_t)o * lpg);
+        C* const gd = dst0 + group_base(grp);
 
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
@@ -510,7 +513,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                     const int k1 = (gq << lvp) + (pos & (VP - 1)), k2 = pos >> lvp;
                     C v = cur[pidx((l << LN) + k1 + (k2 << LVEC))];
                     if constexpr (MF) v = C{v.x * inv_s0, v.y * inv_s0};
-                    stg(dst0 + row_addr(o, gq, q0 + l, pos), v);
+                    stg(gd + off(gq, l, pos), v);
                 }
             } else {
                 cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP>(
@@ -526,7 +529,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                         const int line = (u >> LVEC) >> fast::Log2<nbf>::value;
                         const int gq = k1 >> lvp, fr = k1 & (VP - 1);
                         if constexpr (MF) v = C{v.x * inv_s0, v.y * inv_s0};
-                        stg(dst0 + row_addr(o, gq, q0 + line, fr + (k2 << lvp)), v);
+                        stg(gd + off(gq, line, fr + (k2 << lvp)), v);
                     });
             }
         } else {
@@ -547,7 +550,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                     const int l = e >> LN, rem = e & (N - 1);
                     C v = cur[pidx(e)];
                     if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
-                    stg(dst0 + row_addr(o, rem >> ll, q0 + l, rem & (L - 1)), v);
+                    stg(gd + off(rem >> ll, l, rem & (L - 1)), v);
                 }
             } else {
                 cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP>(
@@ -563,7 +566,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                         const int r = bt & (M - 1), line = bt >> LM;
                         const int gm = r >> g.lb, j = r & (g.b - 1);
                         if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
-                        stg(dst0 + row_addr(o, gm, q0 + line, (j << LVEC) + jj), v);
+                        stg(gd + off(gm, line, (j << LVEC) + jj), v);
                     });
             }
         }

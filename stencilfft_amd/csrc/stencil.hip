@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(256) laplace_kernel(const E* __restrict__ in, 
 #pragma clang fp contract(off)
     using O = EOps<E>;
     constexpr bool REF = sizeof(S) == 8;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= g.field) return;
     const E* src = in + (int64_t)blockIdx.y * g.field;
     const Pt p = decode(g, (int)i);
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E
                                                      int negate) {
 #pragma clang fp contract(off)
     using O = EOps<E>;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= g.field) return;
     const E* src = in + (int64_t)blockIdx.y * g.field;
     const Pt p = decode(g, (int)i);
