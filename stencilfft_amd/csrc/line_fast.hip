@@ -45,6 +45,21 @@ struct NumPasses {
 
 }  // namespace fast
 
+// LDS padding of the line group: one element after every 2^kPadShift, chosen
+// by measurement: lines of N >= 4096 pad every 16 (FP32) or 64 (FP64)
+// elements, shorter lines every 32 (lowest pass times at N = 8192 and 512).
+template <typename T, int LN>
+constexpr int kPadShift = LN >= 12 ? (sizeof(T) == 8 ? 6 : 4) : 5;
+template <typename T, int LN>
+__device__ __forceinline__ int fidx(int i) {
+    return i + (i >> kPadShift<T, LN>);
+}
+template <typename T, int LN>
+__host__ __device__ constexpr int fast_padded(int e) {
+    return e + (e >> kPadShift<T, LN>) + 1;
+}
+
+
 // Twiddle tables for a line of length N = VEC*M (S = max(VEC, M)):
 //   [0, S)          W_S^t
 //   [S, S+64)       W_N^t,      t < 64
@@ -107,7 +122,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
             kk = k1;
         }
 #pragma unroll
-        for (int r = 0; r < R; r++) v[k][r] = src[pidx(base + (j + r * nbf) * ES)];
+        for (int r = 0; r < R; r++) v[k][r] = src[fidx<T, LN>(base + (j + r * nbf) * ES)];
         if (PRE) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -130,7 +145,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
         obase[k] = TOGLOBAL ? oidx : base + oidx * ES;
         if constexpr (OOP && !TOGLOBAL) {
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[pidx(obase[k] + r * NS * ES)] = v[k][r];
+            for (int r = 0; r < R; r++) dst[fidx<T, LN>(obase[k] + r * NS * ES)] = v[k][r];
         }
     }
     if constexpr (OOP && !TOGLOBAL) {
@@ -140,7 +155,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
 #pragma unroll
         for (int k = 0; k < BPT; k++) {
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[pidx(obase[k] + r * NS * ES)] = v[k][r];
+            for (int r = 0; r < R; r++) dst[fidx<T, LN>(obase[k] + r * NS * ES)] = v[k][r];
         }
         __syncthreads();
     } else {
@@ -257,7 +272,7 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int r = 4 * g + i;
-            x[i] = src[pidx(base + (j + r * nbf) * ES)];
+            x[i] = src[fidx<T, LN>(base + (j + r * nbf) * ES)];
             if (PRE) {
                 const int e = ((j + r * nbf) * kk) & (N - 1);
                 if constexpr (FULLTAB) x[i] = ctw<FWD>(x[i], ta[e]);
@@ -298,8 +313,8 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
                 sink(u, ob + s0 * NS, y0);
                 sink(u, ob + (s0 + 1) * NS, y1);
             } else {
-                dst[pidx(ob + s0 * NS * ES)] = y0;
-                dst[pidx(ob + (s0 + 1) * NS * ES)] = y1;
+                dst[fidx<T, LN>(ob + s0 * NS * ES)] = y0;
+                dst[fidx<T, LN>(ob + (s0 + 1) * NS * ES)] = y1;
             }
         }
     }
@@ -371,8 +386,8 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     // store instruction; instead it stays in LDS and a sweep writes whole rows
     constexpr bool STAGED = LVEC == 4;
     C* lds = reinterpret_cast<C*>(smem_raw);
-    C* lds2 = OOP ? lds + padded_elems(E) : lds;
-    C* tabs = lds + padded_elems(E) * (OOP ? 2 : 1);
+    C* lds2 = OOP ? lds + fast_padded<T, LN>(E) : lds;
+    C* tabs = lds + fast_padded<T, LN>(E) * (OOP ? 2 : 1);
     const int tid = threadIdx.x;
 
     // global table layout: [W_S | A (64) | B (N/64) | W_N (N)]
@@ -465,8 +480,8 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 const int gm = rem >> ll, pos = rem & (L - 1);
                 li = (l << LN) + ((pos >> lvp) << LVEC) + (gm << lvp) + (pos & (VP - 1));
             }
-            if constexpr (MF) lds[pidx(li)] = C{pre[k].x * s0, pre[k].y * s0};
-            else lds[pidx(li)] = pre[k];
+            if constexpr (MF) lds[fidx<T, LN>(li)] = C{pre[k].x * s0, pre[k].y * s0};
+            else lds[fidx<T, LN>(li)] = pre[k];
         }
     };
 
@@ -511,7 +526,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                     const int l = e >> LN, rem = e & (N - 1);
                     const int gq = rem >> ll, pos = rem & (L - 1);
                     const int k1 = (gq << lvp) + (pos & (VP - 1)), k2 = pos >> lvp;
-                    C v = cur[pidx((l << LN) + k1 + (k2 << LVEC))];
+                    C v = cur[fidx<T, LN>((l << LN) + k1 + (k2 << LVEC))];
                     if constexpr (MF) v = C{v.x * inv_s0, v.y * inv_s0};
                     stg(gd + off(gq, l, pos), v);
                 }
@@ -548,7 +563,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 for (int k = 0; k < EPT; k++) {
                     const int e = tid + k * NT;
                     const int l = e >> LN, rem = e & (N - 1);
-                    C v = cur[pidx(e)];
+                    C v = cur[fidx<T, LN>(e)];
                     if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
                     stg(gd + off(rem >> ll, l, rem & (L - 1)), v);
                 }
@@ -594,7 +609,9 @@ size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false) {
     const int n = vec * m, s = vec > m ? vec : m;
     const int e = n << lines_per_iter_log2(n);
     const int tab = fulltab ? s + n : s + 64 + (n / 64 > 0 ? n / 64 : 1);
-    return es * (padded_elems(e) * (mf ? 2 : 1) + tab) + (mf ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0);
+    const int shift = n >= 4096 ? (es == 16 ? 6 : 4) : 5;
+    const int padded = e + (e >> shift) + 1;
+    return es * (padded * (mf ? 2 : 1) + tab) + (mf ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0);
 }
 
 }  // namespace
