@@ -225,15 +225,18 @@ __global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E
     out[(int64_t)blockIdx.y * g.field + i] = s;
 }
 
-// 2D Laplacian as a rolling strip: a workgroup owns one column rank cid and
-// T consecutive global rows gr; every thread keeps the rows above, at and
-// below its K = L/256 columns in registers, so each row is read from HBM
-// once per strip.  The current row is staged in LDS with a VEC-wide halo on
-// each side (the neighbour ranks' edge vectors, lane-shifted at the global x
-// edge as laplace.cu:148,160 do), so left = lds[pos] and right = lds[pos + 2 VEC].
-template <typename E, typename S, int K>
-__global__ void __launch_bounds__(256) laplace2d_strip_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g,
-                                                              int rows_per_strip, S scale, int apply_scale) {
+// 5/7-point Laplacian as a rolling strip: a workgroup owns one column rank
+// cid, one plane gd (3D) and T consecutive global rows gr; every thread keeps
+// the rows above, at and below its K = L/256 columns in registers, so each row
+// is read once per strip.  The current row is staged in LDS with a VEC-wide
+// halo on each side (the neighbour ranks' edge vectors, lane-shifted at the
+// global x edge as laplace.cu:148,160 do), so left = lds[pos] and right =
+// lds[pos + 2 VEC].  3D: the rows of planes gd - 1 and gd + 1 are loaded
+// directly; the strips of those planes are grid.x * grid.y (a multiple of 8)
+// blocks away, so they run on the same XCD and the loads hit its L2.
+template <typename E, typename S, int K, bool D3>
+__global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g,
+                                                            int rows_per_strip, S scale, int apply_scale) {
 #pragma clang fp contract(off)
     using O = EOps<E>;
     extern __shared__ unsigned char smem_raw[];
@@ -242,49 +245,78 @@ __global__ void __launch_bounds__(256) laplace2d_strip_kernel(const E* __restric
     const int N = g.n, L = g.l, VEC = g.vec, B = g.b, P = g.p;
     const int cid = blockIdx.y;
     const int g0 = blockIdx.x * rows_per_strip;
-    const int64_t fbase = (int64_t)blockIdx.z * g.field;
+    const int gd = D3 ? (int)(blockIdx.z & (unsigned)(N - 1)) : 0;
+    const int64_t fbase = (int64_t)(D3 ? blockIdx.z >> g.ln : blockIdx.z) * g.field;
     const E* src = in + fbase;
     E* dst = out + fbase;
     const int cl = (cid - 1 + P) & (P - 1), cr = (cid + 1) & (P - 1);
-    // offset of global row gr of column rank c
-    auto row_off = [&](int gr, int c) -> int64_t {
-        gr &= N - 1;
-        const int rid = (gr >> g.lb) & (P - 1);
-        const int lr = ((gr >> (g.lb + g.lp)) << g.lb) + (gr & (B - 1));
-        return (int64_t)((rid << g.lp) + c) * g.local + (int64_t)lr * L;
+    // 2D streams with non-temporal loads; 3D keeps rows cacheable for the
+    // neighbouring planes' strips
+    auto ld = [](const E* p) -> E {
+        if constexpr (D3) return *p;
+        else return ld_nt(p);
+    };
+    // block-cyclic global index -> (rank coordinate, local index)
+    auto split = [&](int gi, int& r, int& li) {
+        gi &= N - 1;
+        r = (gi >> g.lb) & (P - 1);
+        li = ((gi >> (g.lb + g.lp)) << g.lb) + (gi & (B - 1));
+    };
+    // offset of global row gr of plane z in column rank c
+    auto row_off = [&](int z, int gr, int c) -> int64_t {
+        int rid, lr, did = 0, ldp = 0;
+        split(gr, rid, lr);
+        if (D3) split(z, did, ldp);
+        const int id = (((did << g.lp) + rid) << g.lp) + c;
+        return (int64_t)id * g.local + ((int64_t)ldp * L + lr) * L;
     };
     E up[K], ce[K], dn[K];
     {
-        const int64_t ou = row_off(g0 - 1, cid), oc = row_off(g0, cid);
+        const int64_t ou = row_off(gd, g0 - 1, cid), oc = row_off(gd, g0, cid);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            up[k] = ld_nt(src + ou + tid + 256 * k);
-            ce[k] = ld_nt(src + oc + tid + 256 * k);
+            up[k] = ld(src + ou + tid + 256 * k);
+            ce[k] = ld(src + oc + tid + 256 * k);
         }
     }
     for (int t = 0; t < rows_per_strip; t++) {
         const int gr = g0 + t;
-        const int64_t od = row_off(gr + 1, cid);
+        const int64_t od = row_off(gd, gr + 1, cid);
 #pragma unroll
-        for (int k = 0; k < K; k++) dn[k] = ld_nt(src + od + tid + 256 * k);
+        for (int k = 0; k < K; k++) dn[k] = ld(src + od + tid + 256 * k);
+        E fr[K], bk[K];
+        if constexpr (D3) {
+            const int64_t of = row_off(gd - 1, gr, cid), ob = row_off(gd + 1, gr, cid);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                fr[k] = ld(src + of + tid + 256 * k);
+                bk[k] = ld(src + ob + tid + 256 * k);
+            }
+        }
         // stage the current row and its halos
 #pragma unroll
         for (int k = 0; k < K; k++) row[VEC + tid + 256 * k] = ce[k];
         for (int jj = tid; jj < VEC; jj += 256) {
             const int jl = (cid == 0) ? ((jj - 1 + VEC) & (VEC - 1)) : jj;
             const int jr = (cid == P - 1) ? ((jj + 1) & (VEC - 1)) : jj;
-            row[jj] = src[row_off(gr, cl) + (B - 1) * VEC + jl];
-            row[VEC + L + jj] = src[row_off(gr, cr) + jr];
+            row[jj] = src[row_off(gd, gr, cl) + (B - 1) * VEC + jl];
+            row[VEC + L + jj] = src[row_off(gd, gr, cr) + jr];
         }
         __syncthreads();
-        const int64_t oc = row_off(gr, cid);
+        const int64_t oc = row_off(gd, gr, cid);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int pos = tid + 256 * k;
-            E acc = lap_combine<sizeof(S) == 8, O, E, S>(up[k], dn[k], row[pos], row[pos + 2 * VEC], ce[k], ce[k],
-                                                          ce[k], false);
+            E acc;
+            if constexpr (D3)
+                acc = lap_combine<sizeof(S) == 8, O, E, S>(up[k], dn[k], row[pos], row[pos + 2 * VEC], fr[k], bk[k],
+                                                           ce[k], true);
+            else
+                acc = lap_combine<sizeof(S) == 8, O, E, S>(up[k], dn[k], row[pos], row[pos + 2 * VEC], ce[k], ce[k],
+                                                           ce[k], false);
             if (apply_scale) acc = O::mul(acc, scale);
-            st_nt(dst + oc + pos, acc);
+            if constexpr (D3) dst[oc + pos] = acc;
+            else st_nt(dst + oc + pos, acc);
         }
         __syncthreads();
 #pragma unroll
@@ -296,24 +328,37 @@ __global__ void __launch_bounds__(256) laplace2d_strip_kernel(const E* __restric
 }
 
 template <typename E, typename S>
-static hipError_t laplace2d_strip(const E* in, E* out, const Geo& g, int batch, double coef, hipStream_t stream) {
+static hipError_t laplace_strip(const E* in, E* out, const Geo& g, int batch, double coef, hipStream_t stream) {
     const int rows = 64 < g.n ? 64 : g.n;
-    const dim3 grid((unsigned)(g.n / rows), (unsigned)g.p, (unsigned)batch);
+    const bool d3 = g.dims == 3;
+    const dim3 grid((unsigned)(g.n / rows), (unsigned)g.p, (unsigned)(d3 ? batch * g.n : batch));
     const size_t lds = sizeof(E) * (size_t)(g.l + 2 * g.vec);
     const S sc = (S)coef;
     const int apply = coef != 1.0 ? 1 : 0;
+#define ZM_STRIP(KK)                                                                                            \
+    if (d3)                                                                                                     \
+        hipLaunchKernelGGL((laplace_strip_kernel<E, S, KK, true>), grid, dim3(256), lds, stream, in, out, g, rows, \
+                           sc, apply);                                                                          \
+    else                                                                                                        \
+        hipLaunchKernelGGL((laplace_strip_kernel<E, S, KK, false>), grid, dim3(256), lds, stream, in, out, g, rows, \
+                           sc, apply);
     switch (g.l) {
-        case 256: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 1>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
-        case 512: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 2>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
-        case 1024: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 4>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
-        case 2048: hipLaunchKernelGGL((laplace2d_strip_kernel<E, S, 8>), grid, dim3(256), lds, stream, in, out, g, rows, sc, apply); break;
+        case 256: ZM_STRIP(1) break;
+        case 512: ZM_STRIP(2) break;
+        case 1024: ZM_STRIP(4) break;
+        case 2048: ZM_STRIP(8) break;
         default: return hipErrorInvalidValue;
     }
+#undef ZM_STRIP
     return hipGetLastError();
 }
 
-static bool strip_ok(const Geo& g) {
-    return g.dims == 2 && (g.l == 256 || g.l == 512 || g.l == 1024 || g.l == 2048);
+static bool strip_ok(const Geo& g, int batch) {
+    const bool lok = g.l == 256 || g.l == 512 || g.l == 1024 || g.l == 2048;
+    if (g.dims == 2) return lok;
+    // 3D: grid.z = batch * N planes, and grid.x * grid.y a multiple of the 8 XCDs
+    const int strips = g.n / (64 < g.n ? 64 : g.n);
+    return lok && (int64_t)batch * g.n <= 65535 && (strips * g.p) % 8 == 0;
 }
 
 template <typename E, typename S>
@@ -323,8 +368,8 @@ static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int 
     const dim3 grid((unsigned)((g.field + threads - 1) / threads), (unsigned)batch);
     const E* src = static_cast<const E*>(in);
     E* dst = static_cast<E*>(out);
-    if (op == OP_LAPLACE && strip_ok(g)) {
-        return laplace2d_strip<E, S>(src, dst, g, batch, coef, stream);
+    if (op == OP_LAPLACE && strip_ok(g, batch)) {
+        return laplace_strip<E, S>(src, dst, g, batch, coef, stream);
     } else if (op == OP_LAPLACE) {
         hipLaunchKernelGGL((laplace_kernel<E, S>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
                            coef != 1.0 ? 1 : 0);

@@ -155,6 +155,16 @@ def test_poisson_config4_fp32_vs_fp64_fd():
     assert rel_l2(grad64s, grad64) < 1e-9
     loss32 = solvers.poisson_loss(f32, g32, p32, form="spectral")
     (grad32,) = torch.autograd.grad(loss32, f32)
+    # time of one FP32 loss + gradient (warm), for the record
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(5):
+        (gt,) = torch.autograd.grad(solvers.poisson_loss(f32, g32, p32, form="spectral"), f32)
+    ev[1].record()
+    torch.cuda.synchronize()
+    note(f"fp32 loss + gradient, 8 x 4096^2: {ev[0].elapsed_time(ev[1]) / 5:.3f} ms")
+    del gt
     rel = rel_l2(grad32, grad64)
     gmax = (grad32.to(torch.complex128) - grad64).abs().max().item() / grad64.abs().max().item()
     note(f"fp32 vs fp64 adjoint gradient rel L2 {rel:.3e}, max {gmax:.3e}; "
@@ -256,10 +266,16 @@ def test_heat_config5_checkpointed_fp32():
     torch.cuda.reset_peak_memory_stats()
     base = torch.cuda.memory_allocated()
     u0r = u0.clone().requires_grad_(True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
     loss = solvers.sq_norm(solvers.rollout(u0r, step32, steps, every=every) - target)
+    ev[1].record()
     (g32,) = torch.autograd.grad(loss, u0r)
+    ev[2].record()
     torch.cuda.synchronize()
     peak = torch.cuda.max_memory_allocated() - base
+    note(f"forward {steps} steps {ev[0].elapsed_time(ev[1]):.1f} ms, checkpointed backward "
+         f"{ev[1].elapsed_time(ev[2]):.1f} ms")
     state = p32.numel() * 8
     note(f"peak HBM above inputs {peak / 2**30:.2f} GiB = {peak / state:.1f} states of {state / 2**30:.2f} GiB "
          f"(storing every state would take {steps} states)")

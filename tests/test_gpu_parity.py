@@ -222,7 +222,8 @@ def test_ramp_closed_form_on_gpu(oracle_mod):
 # ---------------------------------------------------------------------------
 
 STRIP_SHAPES = [(2, 512, 2, 16), (2, 1024, 4, 8), (2, 1024, 2, 8), (2, 2048, 2, 16), (2, 4096, 2, 32),
-                (2, 8192, 4, 64)]
+                (2, 8192, 4, 64),
+                (3, 512, 2, 16)]
 
 
 @pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D] + STRIP_SHAPES)
@@ -257,7 +258,7 @@ def test_dx_dy_fp64_bit_exact(oracle_mod, n, p, b):
             assert np.array_equal(ax, -gx)
 
 
-@pytest.mark.parametrize("dims,n,p,b", [(2, 256, 4, 8), (3, 32, 2, 4)])
+@pytest.mark.parametrize("dims,n,p,b", [(2, 256, 4, 8), (3, 32, 2, 4), (3, 512, 2, 16)])
 def test_stencils_fp32(oracle_mod, dims, n, p, b):
     note(f"fp32 stencils {dims}D N={n}")
     s = oracle_mod.Shape(dims, n, p, b)
