@@ -327,10 +327,19 @@ __global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict_
     }
 }
 
+// Rows per strip: 64 (each row read once, two halo rows per strip), fewer
+// for small 2D fields so that the grid still has about 512 workgroups.
+static int strip_rows(const Geo& g, int batch) {
+    int rows = 64 < g.n ? 64 : g.n;
+    if (g.dims == 2)
+        while (rows > 4 && (int64_t)(g.n / rows) * g.p * batch < 512) rows >>= 1;
+    return rows;
+}
+
 template <typename E, typename S>
 static hipError_t laplace_strip(const E* in, E* out, const Geo& g, int batch, double coef, hipStream_t stream) {
-    const int rows = 64 < g.n ? 64 : g.n;
     const bool d3 = g.dims == 3;
+    const int rows = strip_rows(g, batch);
     const dim3 grid((unsigned)(g.n / rows), (unsigned)g.p, (unsigned)(d3 ? batch * g.n : batch));
     const size_t lds = sizeof(E) * (size_t)(g.l + 2 * g.vec);
     const S sc = (S)coef;
@@ -357,7 +366,7 @@ static bool strip_ok(const Geo& g, int batch) {
     const bool lok = g.l == 256 || g.l == 512 || g.l == 1024 || g.l == 2048;
     if (g.dims == 2) return lok;
     // 3D: grid.z = batch * N planes, and grid.x * grid.y a multiple of the 8 XCDs
-    const int strips = g.n / (64 < g.n ? 64 : g.n);
+    const int strips = g.n / strip_rows(g, batch);
     return lok && (int64_t)batch * g.n <= 65535 && (strips * g.p) % 8 == 0;
 }
 
