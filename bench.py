@@ -52,6 +52,10 @@ def run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream):
 
 
 def time_path(plan, x, n, steps, warmup, torch, label):
+    """Time `steps` steps with the plan's per-launch HIP events on (recorded on
+    the launch stream, inside the timed region: the stage breakdown and the
+    roofline come from them), then the same steps with no per-launch events
+    (the cost of that instrumentation)."""
     lap = torch.empty_like(x)
     y = torch.empty_like(x)
     z = torch.empty_like(x)
@@ -59,24 +63,60 @@ def time_path(plan, x, n, steps, warmup, torch, label):
     stream = torch.cuda.current_stream()
     run_steps(plan, x, lap, y, z, scratch, n, warmup, torch, stream)
     torch.cuda.synchronize()
-    plan.timing_reset()
-    plan.timing(True)
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    plan.timing(False)
-    ms = start.elapsed_time(stop) / steps
+
+    def timed(instrumented):
+        if instrumented:
+            plan.timing_reset()
+            plan.timing(True)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record(stream)
+        run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if instrumented:
+            plan.timing(False)
+        return start.elapsed_time(stop) / steps, wall
+
+    ms, wall = timed(True)
     stages = plan.timing_read()
     plan.timing_reset()
-    log(f"{label}: {ms:.3f} ms/step (wall {1e3 * wall / steps:.3f} ms/step); stages " +
-        ", ".join(f"{k}: {c} x {t / c:.3f} ms" for k, (c, t) in stages.items()))
+    ms_plain, _ = timed(False)
+    log(f"{label}: {ms:.3f} ms/step (wall {1e3 * wall / steps:.3f} ms/step, without per-launch events "
+        f"{ms_plain:.3f}); stages " + ", ".join(f"{k}: {c} x {t / c:.3f} ms" for k, (c, t) in stages.items()))
+    time_path.last_plain = ms_plain
     return ms, stages, (lap, y, z)
+
+
+def time_graph(plan, x, n, steps, torch):
+    """One step captured in a HIP graph (torch.cuda.graph on the plan's
+    stream-ordered launches), replayed `steps` times."""
+    lap, y, z = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    scratch = torch.empty(plan.scratch_bytes // x.element_size(), dtype=x.dtype, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run_steps(plan, x, lap, y, z, scratch, n, 2, torch, None)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        run_steps(plan, x, lap, y, z, scratch, n, 1, torch, None)
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        g.replay()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    log(f"graph replay: {ms:.4f} ms/step")
+    return ms
 
 
 def cpu_baseline(dims, n, p, b, seed):
@@ -165,6 +205,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=8192)
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the step captured once in a HIP graph and replayed (graph_ms_per_step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device-to-device copy of one N^2 complex64 field (the HBM reference "
                          "kernel the rocprof FETCH_SIZE / WRITE_SIZE counters are scaled by) and exit")
@@ -266,7 +308,11 @@ def main():
         "bytes_per_launch_s8d": s8d_per_launch,
     }
 
-    result_extra = {
+    if args.graph:
+        result_extra_graph = {"graph_ms_per_step": round(time_graph(plan, x, n, args.steps, torch), 4)}
+    else:
+        result_extra_graph = {}
+    result_extra = {**result_extra_graph, "ms_per_step_without_launch_events": round(time_path.last_plain, 4),
         "stage_ms": {k: round(t / c, 4) for k, (c, t) in stages.items()},
         "stage_launches_per_step": {k: c // args.steps for k, (c, t) in stages.items()},
         "step_bytes_s8d": bytes_s8d,
