@@ -123,8 +123,8 @@ def test_stencil_symbol_equals_stencil_kernels(dims, n, p, b):
     assert abs(lhs - rhs) < 1e-12 * abs(lhs)
 
 
-def test_poisson_config4_fp32_vs_fp64_fd():
-    """Configuration 4: batch of 8 N=4096 fields (P=4, B=16), FP32 solve and
+def test_poisson_batch8_n4096_fp32_gradient():
+    """Batch of 8 N=4096 fields (P=4, B=16), FP32 solve and
     hand-written backward (stencil symbols fused with the FP64 Poisson
     multiplier), checked against FP64 central differences (eps = 1e-6) on 32
     random coordinates and the FP64 adjoint gradient of the stencil-kernel
@@ -132,7 +132,7 @@ def test_poisson_config4_fp32_vs_fp64_fd():
     conditioning of the N^2-scaled Laplacian on FP32-stored u, not the kernels
     (each FP32 operator alone is within ~1e-6 of FP64 on the same input)."""
     from stencilfft_amd import solvers
-    note("config 4 poisson: 8 x 4096^2 fp32")
+    note("poisson gradient: 8 x 4096^2 fp32")
     n, p, b, batch = 4096, 4, 16, 8
     p32 = plan(2, n, p, b, "fp32", batch)
     p64 = plan(2, n, p, b, "fp64", batch)
@@ -236,8 +236,8 @@ def test_heat_rollout_checkpointed_gradient(dims, n, p, b):
     assert np.max(np.abs(fd - gc)) < 1e-5 * np.max(np.abs(gc))
 
 
-def test_heat_config5_checkpointed_fp32():
-    """Configuration 5: batch of 16 fields of 256^3 complex FP32 (P=2, B=8),
+def test_heat_batch16_256cubed_checkpointed_fp32():
+    """Batch of 16 fields of 256^3 complex FP32 (P=2, B=8),
     32 steps of u <- IFFT(exp(-nu |k|^2 dt) FFT(u)) + dt Lap7(u), nu = 1e-3,
     dt = 1e-2, initial condition from 128 random modes; gradient of the
     final-state L2 misfit w.r.t. the initial condition through the adjoint
@@ -249,7 +249,7 @@ def test_heat_config5_checkpointed_fp32():
     from stencilfft_amd.fields import mode_fields
     n, p, b, batch, steps, every = 256, 2, 8, 16, 32, 8
     nu, dt = 1e-3, 1e-2
-    note(f"config 5 heat: {batch} x {n}^3 fp32, {steps} steps, checkpoints every {every}")
+    note(f"heat rollout gradient: {batch} x {n}^3 fp32, {steps} steps, checkpoints every {every}")
     p32 = plan(3, n, p, b, "fp32", batch)
     p64_1 = plan(3, n, p, b, "fp64", 1)
     u0_64 = torch.empty(p32.numel(), dtype=torch.complex128, device="cuda")

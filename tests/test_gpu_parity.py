@@ -174,9 +174,9 @@ def test_fast_line_pass_vs_oracle(oracle_mod, dims, n, p, b, precision):
         assert rel_l2(z, zwant) < 1e-6 and rel_max(z, zwant) < 1e-5
 
 
-def test_config1_shape_fp64(oracle_mod):
-    # N_DIM=1024, P_DIM=2, B_DIM=16 complex128, re/im U[-1,1) -- the plain small 2D configuration
-    note("C1 N=1024 P=2 B=16 fp64")
+def test_n1024_p2_b16_fp64_round_trip(oracle_mod):
+    # N_DIM=1024, P_DIM=2, B_DIM=16 complex128, re/im U[-1,1): forward, inverse, Laplacian
+    note("N=1024 P=2 B=16 fp64")
     n, p, b = 1024, 2, 16
     s = oracle_mod.Shape(2, n, p, b)
     nat = uniform_field((n, n), 1)
