@@ -97,6 +97,13 @@ zm_status zm_plan_destroy(zm_plan* plan);
 
 zm_status zm_plan_dims(const zm_plan* plan, zm_shape* shape, zm_dims* dims, zm_precision* precision, int* batch);
 
+/* The kernel path a plan runs (no reference counterpart): a bitwise OR of
+ * the flags below. */
+#define ZM_PATH_TWO_STAGE 1u   /* two kernels per direction (post-exchange buffer materialised) */
+#define ZM_PATH_FAST_LINE 2u   /* compile-time specialised fused line pass */
+#define ZM_PATH_MFMA 4u        /* radix-16 stages as split-precision MFMA GEMMs */
+zm_status zm_plan_path(const zm_plan* plan, unsigned* path);
+
 /* Bytes of device scratch the transforms need (0 when none). */
 size_t zm_scratch_bytes(const zm_plan* plan);
 
@@ -236,7 +243,7 @@ zm_status zm_packed_to_natural(zm_plan* plan, const void* packed, void* natural,
 
 /* Stage kinds timed by zm_timing_*.  Replaces the per-stage wall-clock
  * timers of __PRINT__DETAILED__TIMING__ (TIME_MPI_MAX_NS,
- * zmodel/transforms/fft.cu:8-21, 207-328) with HIP events recorded around
+ * zmodel/transforms/fft.cu:8-21, 207-328) with HIP events recorded after
  * every kernel launch on the caller's stream. */
 typedef enum {
     ZM_STAGE_LINE = 0,     /* fused FFT1a + exchange + twiddle + FFT1b pass */
@@ -250,8 +257,10 @@ typedef enum {
     ZM_STAGE_COUNT = 8
 } zm_stage;
 
-/* enable != 0 starts recording an event pair around every subsequent kernel
- * launch of this plan (not thread-safe; meant for benchmarks); 0 stops. */
+/* enable != 0 starts recording one event after every subsequent kernel launch
+ * of this plan on its stream; a launch's time runs from the previous timed
+ * launch's event on the same stream (an event recorded just before it for
+ * the first) to its own.  Not thread-safe; meant for benchmarks.  0 stops. */
 zm_status zm_timing_enable(zm_plan* plan, int enable);
 /* Waits for the recorded events and returns the number of launches of
  * `stage` and their summed duration in milliseconds. */

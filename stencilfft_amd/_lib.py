@@ -27,6 +27,7 @@ STATUS_NAMES = {
 
 ZM_FP64, ZM_FP32, ZM_FP32_MFMA = 0, 1, 2
 ZM_FLAG_TWO_STAGE = 1
+ZM_PATH_TWO_STAGE, ZM_PATH_FAST_LINE, ZM_PATH_MFMA = 1, 2, 4
 ZM_COMPLEX, ZM_REAL = 0, 1
 ZM_SPECTRAL_POISSON, ZM_SPECTRAL_HEAT, ZM_SPECTRAL_SCALE, ZM_SPECTRAL_STENCIL, ZM_SPECTRAL_POISSON_STENCIL = range(5)
 STAGES = ["line", "stage_a", "stage_b", "repack", "laplace", "deriv", "spectral", "convert"]
@@ -64,6 +65,7 @@ SIGNATURES = {
     "zm_plan_create": (_I, [ctypes.POINTER(_VP), _SP, _I, _I, ctypes.c_uint]),
     "zm_plan_destroy": (_I, [_VP]),
     "zm_plan_dims": (_I, [_VP, _SP, ctypes.POINTER(ZmDims), _IP, _IP]),
+    "zm_plan_path": (_I, [_VP, ctypes.POINTER(ctypes.c_uint)]),
     "zm_scratch_bytes": (ctypes.c_size_t, [_VP]),
     "zm_field_bytes": (ctypes.c_size_t, [_VP]),
     "zm_forward": (_I, [_VP, _VP, _VP, _VP, _VP]),

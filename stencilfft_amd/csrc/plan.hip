@@ -40,27 +40,42 @@ struct zm_plan {
     struct Timed {
         int stage;
         hipEvent_t start, stop;
+        bool own_start;  // start is this entry's own event, not the previous launch's stop
     };
     std::vector<Timed> timed;
+    hipEvent_t last_stop;  // completion event of the previous timed launch
+    hipStream_t last_stream;
 };
 
 namespace {
 
-// Runs `launch` (returning hipError_t), bracketed by an event pair on
-// `stream` when the plan records per-stage timing.
+// Runs `launch` (returning hipError_t) and, when the plan records per-stage
+// timing, records one event after it on `stream`.  A launch's time is the
+// interval from the previous timed launch's completion event on the same
+// stream (or from an event recorded just before it, for the first launch on
+// a stream) to its own: one event per launch, so back-to-back launches are
+// timed without doubling the events in the queue.
 template <typename F>
 hipError_t timed_launch(zm_plan* p, int stage, hipStream_t stream, F&& launch) {
     if (!p->timing) return launch();
-    zm_plan::Timed t{stage, nullptr, nullptr};
-    hipError_t e = hipEventCreate(&t.start);
+    zm_plan::Timed t{stage, nullptr, nullptr, false};
+    hipError_t e = hipSuccess;
+    if (p->last_stop && p->last_stream == stream) {
+        t.start = p->last_stop;
+    } else {
+        e = hipEventCreate(&t.start);
+        if (e == hipSuccess) e = hipEventRecord(t.start, stream);
+        t.own_start = true;
+    }
     if (e == hipSuccess) e = hipEventCreate(&t.stop);
-    if (e == hipSuccess) e = hipEventRecord(t.start, stream);
     if (e == hipSuccess) e = launch();
     if (e == hipSuccess) e = hipEventRecord(t.stop, stream);
     if (e == hipSuccess) {
         p->timed.push_back(t);
+        p->last_stop = t.stop;
+        p->last_stream = stream;
     } else {
-        if (t.start) (void)hipEventDestroy(t.start);
+        if (t.own_start && t.start) (void)hipEventDestroy(t.start);
         if (t.stop) (void)hipEventDestroy(t.stop);
     }
     return e;
@@ -68,10 +83,12 @@ hipError_t timed_launch(zm_plan* p, int stage, hipStream_t stream, F&& launch) {
 
 void drop_timing(zm_plan* p) {
     for (auto& t : p->timed) {
-        (void)hipEventDestroy(t.start);
+        if (t.own_start) (void)hipEventDestroy(t.start);
         (void)hipEventDestroy(t.stop);
     }
     p->timed.clear();
+    p->last_stop = nullptr;
+    p->last_stream = nullptr;
 }
 
 }  // namespace
@@ -270,6 +287,13 @@ zm_status zm_plan_dims(const zm_plan* p, zm_shape* shape, zm_dims* dims, zm_prec
     if (dims) *dims = p->dims;
     if (precision) *precision = p->precision;
     if (batch) *batch = p->batch;
+    return ZM_OK;
+}
+
+zm_status zm_plan_path(const zm_plan* p, unsigned* path) {
+    if (!p || !path) return ZM_ERR_INVALID_ARG;
+    *path = (p->two_stage ? ZM_PATH_TWO_STAGE : 0u) | (p->fast ? ZM_PATH_FAST_LINE : 0u) |
+            (p->precision == ZM_FP32_MFMA ? ZM_PATH_MFMA : 0u);
     return ZM_OK;
 }
 
@@ -673,6 +697,7 @@ void zm_global_to_reciprocal_3d(const zm_shape* s, int kd, int kr, int kc, int* 
 zm_status zm_timing_enable(zm_plan* p, int enable) {
     if (!p) return ZM_ERR_INVALID_ARG;
     p->timing = enable != 0;
+    p->last_stop = nullptr;  // a new timed sequence starts with its own start event
     return ZM_OK;
 }
 

@@ -128,6 +128,11 @@ class Plan:
                                        batch, flags), f"zm_plan_create({self.shape}, {precision})")
         self._h = h
         self.scratch_bytes = L._lib().zm_scratch_bytes(h)
+        path = ctypes.c_uint()
+        L.check(L._lib().zm_plan_path(h, ctypes.byref(path)), "zm_plan_path")
+        self.two_stage = bool(path.value & L.ZM_PATH_TWO_STAGE)
+        self.fast = bool(path.value & L.ZM_PATH_FAST_LINE)
+        self.mfma = bool(path.value & L.ZM_PATH_MFMA)
 
     # ---- lifetime ----
     def close(self):

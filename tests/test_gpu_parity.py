@@ -494,3 +494,13 @@ def test_graph_capture_replay(dims, n, p, b, precision):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(z, want), f"replay {i}"
+
+
+def test_plan_path_query():
+    note("plan path")
+    assert plan(2, 8192, 4, 32, "fp32").fast
+    assert plan(2, 1024, 2, 16, "fp32_mfma").mfma
+    p2 = plan(2, 1024, 2, 16, "fp64", two_stage=True)
+    assert p2.two_stage and not p2.fast
+    p3 = plan(2, 64, 2, 2, "fp32")  # M = 4: generic line pass
+    assert not p3.fast and not p3.two_stage
