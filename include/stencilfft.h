@@ -41,7 +41,8 @@ typedef enum {
     ZM_ERR_DEVICE = 4,      /* no HIP device, or a HIP runtime call failed */
     ZM_ERR_ALIAS = 5,       /* in and out alias where that is not allowed */
     ZM_ERR_LAUNCH = 6,      /* a kernel launch failed */
-    ZM_ERR_NOMEM = 7        /* device allocation failed */
+    ZM_ERR_NOMEM = 7,       /* device allocation failed */
+    ZM_ERR_IO = 8           /* file I/O failed (stencilfft_hdf5.h) */
 } zm_status;
 
 typedef enum {

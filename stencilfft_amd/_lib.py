@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libstencilfft.so")
 
 ZM_OK = 0
 (ZM_ERR_INVALID_ARG, ZM_ERR_SHAPE, ZM_ERR_UNSUPPORTED, ZM_ERR_DEVICE, ZM_ERR_ALIAS, ZM_ERR_LAUNCH,
- ZM_ERR_NOMEM) = range(1, 8)
+ ZM_ERR_NOMEM, ZM_ERR_IO) = range(1, 9)
 STATUS_NAMES = {
     0: "ZM_OK",
     1: "ZM_ERR_INVALID_ARG",
@@ -23,6 +23,7 @@ STATUS_NAMES = {
     5: "ZM_ERR_ALIAS",
     6: "ZM_ERR_LAUNCH",
     7: "ZM_ERR_NOMEM",
+    8: "ZM_ERR_IO",
 }
 
 ZM_FP64, ZM_FP32, ZM_FP32_MFMA = 0, 1, 2

@@ -71,6 +71,7 @@ def build(verbose: bool = True) -> str:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB_PATH)
     build_example()
+    build_hdf5()
     if verbose:
         print(f"built {LIB_PATH}")
     return LIB_PATH
@@ -89,6 +90,42 @@ def build_example() -> str:
     if res.returncode != 0:
         raise RuntimeError(f"example build failed:\n{res.stdout}\n{res.stderr}")
     return EXAMPLE_BIN
+
+
+HDF5_SRC = os.path.join(CSRC, "hdf5_io.cpp")
+HDF5_LIB = os.path.join(HERE, "libstencilfft_hdf5.so")
+
+
+def hdf5_prefix() -> str | None:
+    """Install prefix of the serial HDF5 C library: $HDF5_HOME, else the first
+    of /opt/conda, /usr/local, /usr holding include/hdf5.h."""
+    cands = [os.environ.get("HDF5_HOME", ""), "/opt/conda", "/usr/local", "/usr"]
+    for c in cands:
+        if c and os.path.exists(os.path.join(c, "include", "hdf5.h")):
+            return c
+    return None
+
+
+def build_hdf5() -> str | None:
+    """libstencilfft_hdf5.so: the optional HDF5 field I/O (host code only, g++).
+    Skipped, with a message, when no HDF5 installation is found."""
+    prefix = hdf5_prefix()
+    if prefix is None:
+        sys.stderr.write("HDF5 not found (set HDF5_HOME): libstencilfft_hdf5.so not built\n")
+        return None
+    include = os.path.join(os.path.dirname(HERE), "include")
+    libdir = os.path.join(prefix, "lib")
+    tmp = HDF5_LIB + ".tmp"
+    # static libstdc++: the HDF5 prefix may ship an older libstdc++.so that the
+    # rpath below would otherwise put ahead of the one the HIP runtime needs
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-shared", "-fPIC", "-static-libstdc++", "-static-libgcc",
+           HDF5_SRC, f"-I{include}", f"-I{os.path.join(prefix, 'include')}", f"-L{HERE}", "-lstencilfft", f"-L{libdir}", "-lhdf5",
+           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{libdir}", "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"HDF5 I/O build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, HDF5_LIB)
+    return HDF5_LIB
 
 
 if __name__ == "__main__":

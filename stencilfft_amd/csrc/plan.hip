@@ -762,6 +762,7 @@ const char* zm_strerror(zm_status s) {
         case ZM_ERR_ALIAS: return "input and output buffers alias";
         case ZM_ERR_LAUNCH: return "kernel launch failed";
         case ZM_ERR_NOMEM: return "device allocation failed";
+        case ZM_ERR_IO: return "file I/O failed";
     }
     return "unknown status";
 }

@@ -33,8 +33,9 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_strerror():
     from stencilfft_amd import _lib, version
     assert "gfx950" in version()
-    for st in range(8):
+    for st in range(9):
         assert _lib._lib().zm_strerror(st)
+    assert _lib._lib().zm_strerror(_lib.ZM_ERR_IO) == b"file I/O failed"
 
 
 @pytest.mark.parametrize("dims,n,p,b,status", [
