@@ -16,6 +16,8 @@ Prints one JSON line on stdout (diagnostics go to stderr):
               host cores, one full step of the same field
   errors      relative L2 / max error of every FP32 output against the FP64
               GPU path on the same FP32-rounded input
+  library_fft the FFT round trip alone beside torch.fft.fftn + ifftn (rocFFT)
+              on the natural layout of the same field and precision
 """
 from __future__ import annotations
 
@@ -120,6 +122,59 @@ def time_graph(plan, x, n, steps, torch):
     return ms
 
 
+def library_fft_comparison(plan, x, n, steps, warmup, torch):
+    """The FFT round trip alone (forward, then inverse scaled 1/N^d) on the
+    packed layout against torch.fft.fftn / ifftn (hipFFT over rocFFT) on the
+    natural layout of the same field and precision: the role of the
+    reference's heFFTe + cuFFT example (heffte/heffte_example_cuda.cpp).
+    Forward errors of both are taken against the FP64 library transform."""
+    dims = plan.shape.dims
+    axes = tuple(range(dims))
+    nat = plan.packed_to_natural(x).reshape((n,) * dims)
+    y, z = torch.empty_like(x), torch.empty_like(x)
+    scratch = torch.empty(plan.scratch_bytes // x.element_size(), dtype=x.dtype, device="cuda")
+    stream = torch.cuda.current_stream()
+    scale = 1.0 / float(n) ** dims
+
+    def ours(k):
+        for _ in range(k):
+            plan.forward(x, out=y, scratch=scratch, stream=stream)
+            plan.inverse(y, scale, out=z, scratch=scratch, stream=stream)
+
+    def lib(k):
+        for _ in range(k):
+            f = torch.fft.fftn(nat, dim=axes)
+            torch.fft.ifftn(f, dim=axes)
+
+    out = {}
+    for name, fn in (("ours", ours), ("library", lib)):
+        fn(warmup)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        fn(steps)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        out[name] = ev0.elapsed_time(ev1) / steps
+    ref = torch.fft.fftn(nat.to(torch.complex128), dim=axes).reshape(-1)
+    tab = plan.reciprocal_index_table()
+    ours_nat = torch.empty_like(y)
+    ours_nat[tab] = y
+    del tab
+    err_ours = rel_errors(ours_nat, ref, torch)
+    del ours_nat
+    err_lib = rel_errors(torch.fft.fftn(nat, dim=axes).reshape(-1), ref, torch)
+    res = {
+        "library": f"torch.fft.fftn + ifftn (hipFFT/rocFFT), natural {n}^{dims} layout, {str(x.dtype)[6:]}",
+        "ms_per_round_trip": round(out["library"], 4),
+        "ours_ms_per_round_trip": round(out["ours"], 4),
+        "speedup_vs_library": round(out["library"] / out["ours"], 3),
+        "forward_errors_vs_fp64_library": {"ours": err_ours, "library": err_lib},
+    }
+    log(f"library FFT comparison: {json.dumps(res)}")
+    return res
+
+
 def cpu_baseline(dims, n, p, b, seed):
     """FP64 CPU oracle (C port of the reference path, radix-2 DFT mode, OpenMP
     over the host cores) on one full step of the same shape, and beside it a
@@ -205,6 +260,8 @@ def main():
     ap.add_argument("--two-stage", action="store_true", help="force the two-kernel-per-direction path")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-library-fft", action="store_true",
+                    help="skip the torch.fft (rocFFT) natural-layout round-trip comparison")
     ap.add_argument("--seed", type=int, default=8192)
     ap.add_argument("--graph", action="store_true",
                     help="also time the step captured once in a HIP graph and replayed (graph_ms_per_step)")
@@ -348,6 +405,12 @@ def main():
             result_extra["speedup_vs_fp64_two_stage_gpu"] = round(ms64s / ms, 3)
             del outs64s, plan64s
         del x64r
+
+    if not args.no_library_fft:
+        del outs
+        torch.cuda.empty_cache()
+        result_extra["library_fft"] = library_fft_comparison(plan, x, n, max(3, min(args.steps, 10)), 2, torch)
+        outs = None
 
     cpu = None
     if not args.no_cpu_baseline:
