@@ -156,15 +156,24 @@ typedef enum { ZM_COMPLEX = 0, ZM_REAL = 1 } zm_element;
 zm_status zm_laplace(zm_plan* plan, const void* in, void* out, double scale, zm_element element, void* stream);
 
 /* Periodic 4th-order central differences [1, -8, 0, 8, -1] / (12 delta) along
- * x (packed columns) or y (block-cyclic rows); 2D, B >= 2.  Replaces dx / dy
- * with their halo packers (zmodel/stencils/dx.cu:10-161, dy.cu:10-185); the
- * complex form applies the real stencil to re and im. */
+ * x (packed columns) or y (block-cyclic rows); B >= 2.  Replaces dx / dy
+ * with their halo packers (zmodel/stencils/dx.cu:10-161, dy.cu:10-185, which
+ * are 2D); on a 3D plan they act on every plane gd.  The complex form applies
+ * the real stencil to re and im. */
 zm_status zm_dx4(zm_plan* plan, const void* in, void* out, double delta, zm_element element, void* stream);
 zm_status zm_dy4(zm_plan* plan, const void* in, void* out, double delta, zm_element element, void* stream);
 
-/* Transposed stencils: dx4^T = -dx4, dy4^T = -dy4 (no reference counterpart). */
+/* The same stencil along z (block-cyclic depth); 3D plans only, B >= 2
+ * (ZM_ERR_UNSUPPORTED otherwise).  No reference counterpart: the reference
+ * has dx and dy only; its depth axis is distributed like its rows
+ * (zmodel/utils.h:193-199). */
+zm_status zm_dz4(zm_plan* plan, const void* in, void* out, double delta, zm_element element, void* stream);
+
+/* Transposed stencils: dx4^T = -dx4, dy4^T = -dy4, dz4^T = -dz4 (no
+ * reference counterpart). */
 zm_status zm_dx4_adjoint(zm_plan* plan, const void* in, void* out, double delta, zm_element element, void* stream);
 zm_status zm_dy4_adjoint(zm_plan* plan, const void* in, void* out, double delta, zm_element element, void* stream);
+zm_status zm_dz4_adjoint(zm_plan* plan, const void* in, void* out, double delta, zm_element element, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* reciprocal-space multipliers (no reference counterpart; built on        */

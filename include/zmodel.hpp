@@ -167,6 +167,12 @@ inline void dy(FftHostPlans& p, const Scalar* device_in, Scalar* device_out, Sca
     check(zm_dy4(p.plan, device_in, device_out, delta, ZM_REAL, stream), "dy");
 }
 
+// dz: the same stencil along depth on a 3D plan (no reference counterpart)
+inline void dz(FftHostPlans& p, const Scalar* device_in, Scalar* device_out, Scalar delta,
+               hipStream_t stream = nullptr) {
+    check(zm_dz4(p.plan, device_in, device_out, delta, ZM_REAL, stream), "dz");
+}
+
 // ---------------------------------------------------------------------------
 // utils.h
 // ---------------------------------------------------------------------------

@@ -318,6 +318,35 @@ class Shape:
             raise ValueError("dy needs a 2D shape with B >= 2")
         return out
 
+    def deriv4(self, x, axis: str, delta=1.0):
+        """dx / dy / dz of a packed grid (axis "x", "y" or "z"), 2D or 3D, by
+        way of the natural layout and ``deriv4_natural``."""
+        if self.b < 2 or (axis == "z" and self.dims != 3):
+            raise ValueError("deriv4 needs B >= 2, and dz a 3D shape")
+        nat_axis = {"x": self.dims - 1, "y": self.dims - 2, "z": 0}[axis]
+        return self.natural_to_real(deriv4_natural(self.real_to_natural(x), nat_axis, delta))
+
+
+def deriv4_natural(f: np.ndarray, axis: int, delta: float = 1.0) -> np.ndarray:
+    """Periodic 4th-order central difference along `axis` of a natural-layout
+    field, summed in the reference's order (zmodel/stencils/dx.cu:29-79,
+    dy.cu:34-97): s = 0 + f(-1) * -8 + f(-2) + f(+1) * 8 - f(+2), then
+    s / (12 delta), each operation rounded once.  Complex fields: the real
+    stencil on re and im separately.  In 2D this is bit-identical to the C
+    restatement of dx / dy; along depth (no reference counterpart) it is the
+    same stencil on the third axis."""
+    if np.iscomplexobj(f):
+        out = np.empty_like(f)
+        out.real = deriv4_natural(np.ascontiguousarray(f.real), axis, delta)
+        out.imag = deriv4_natural(np.ascontiguousarray(f.imag), axis, delta)
+        return out
+    s = np.zeros_like(f)
+    s = s + np.roll(f, 1, axis) * -8.0
+    s = s + np.roll(f, 2, axis)
+    s = s + np.roll(f, -1, axis) * 8.0
+    s = s - np.roll(f, -2, axis)
+    return s / (12.0 * delta)
+
 
 def set_dft_mode(mode: int) -> None:
     """0 = direct DFT (definition, long-double accumulation), 1 = radix-2 FFT."""

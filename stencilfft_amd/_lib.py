@@ -80,6 +80,8 @@ SIGNATURES = {
     "zm_dy4": (_I, [_VP, _VP, _VP, _D, _I, _VP]),
     "zm_dx4_adjoint": (_I, [_VP, _VP, _VP, _D, _I, _VP]),
     "zm_dy4_adjoint": (_I, [_VP, _VP, _VP, _D, _I, _VP]),
+    "zm_dz4": (_I, [_VP, _VP, _VP, _D, _I, _VP]),
+    "zm_dz4_adjoint": (_I, [_VP, _VP, _VP, _D, _I, _VP]),
     "zm_spectral_multiply": (_I, [_VP, _VP, _VP, _I, _D, _D, _D, _VP]),
     "zm_spectral_multiply_adjoint": (_I, [_VP, _VP, _VP, _I, _D, _D, _D, _VP]),
     "zm_packed_to_block_cyclic": (_I, [_SP, _I]),

@@ -5,7 +5,8 @@ adjoint, computed by the library's adjoint entry points:
   forward FFT          -> zm_forward_adjoint  (the inverse chain, scale 1)
   inverse FFT (s)      -> zm_inverse_adjoint  (s x the forward chain)
   Laplacian            -> itself (symmetric periodic stencil)
-  dx4 / dy4            -> zm_dx4_adjoint / zm_dy4_adjoint (the transposed stencil, -dx4)
+  dx4 / dy4 / dz4      -> zm_dx4_adjoint / zm_dy4_adjoint / zm_dz4_adjoint (the
+                          transposed stencil, -dx4)
   spectral multiplier  -> zm_spectral_multiply_adjoint (conjugate multiplier;
                           the real kinds are self-adjoint)
 For complex tensors torch passes dL/d(conj y) and expects A^H of it back, which
@@ -71,6 +72,17 @@ class _Dy4(torch.autograd.Function):
         return ctx.plan.dy4(g.contiguous(), ctx.delta, adjoint=True), None, None
 
 
+class _Dz4(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, plan, delta):
+        ctx.plan, ctx.delta = plan, delta
+        return plan.dz4(x.contiguous(), delta)
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.plan.dz4(g.contiguous(), ctx.delta, adjoint=True), None, None
+
+
 class _Spectral(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, plan, kind, p0, p1, length):
@@ -106,6 +118,10 @@ def dx4(x, plan, delta=1.0):
 
 def dy4(x, plan, delta=1.0):
     return _Dy4.apply(x, plan, float(delta))
+
+
+def dz4(x, plan, delta=1.0):
+    return _Dz4.apply(x, plan, float(delta))
 
 
 def spectral(x, plan, kind, p0=0.0, p1=0.0, length=1.0):

@@ -60,7 +60,7 @@ int line_table_entries(int vec, int m);
 hipError_t launch_line_tables(double2* t64, float2* t32, int vec, int m, hipStream_t stream);
 
 // stencils: E = element type (T or Cx<T>)
-enum StencilOp { OP_LAPLACE = 0, OP_DX4 = 1, OP_DY4 = 2 };
+enum StencilOp { OP_LAPLACE = 0, OP_DX4 = 1, OP_DY4 = 2, OP_DZ4 = 3 };
 template <typename T>
 hipError_t launch_stencil(const void* in, void* out, const Geo& g, int batch, StencilOp op, bool complex_elems,
                           double coef, bool negate, hipStream_t stream);

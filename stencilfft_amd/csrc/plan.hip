@@ -549,7 +549,8 @@ zm_status zm_laplace(zm_plan* p, const void* in, void* out, double scale, zm_ele
 static zm_status deriv(zm_plan* p, const void* in, void* out, double delta, zm_element element, void* stream,
                        zm::StencilOp op, bool negate) {
     if (!p) return ZM_ERR_INVALID_ARG;
-    if (p->shape.dims != 2 || p->shape.b < 2) return ZM_ERR_UNSUPPORTED;  // dx.cu:94-100, dy.cu:117-124 need B >= 2
+    if (p->shape.b < 2) return ZM_ERR_UNSUPPORTED;  // dx.cu:94-100, dy.cu:117-124 need B >= 2
+    if (op == zm::OP_DZ4 && p->shape.dims != 3) return ZM_ERR_UNSUPPORTED;
     if (!(delta != 0.0) || !std::isfinite(delta)) return ZM_ERR_INVALID_ARG;
     return run_stencil(p, in, out, op, 12 * delta, negate, element, stream);
 }
@@ -565,6 +566,12 @@ zm_status zm_dx4_adjoint(zm_plan* p, const void* in, void* out, double delta, zm
 }
 zm_status zm_dy4_adjoint(zm_plan* p, const void* in, void* out, double delta, zm_element element, void* stream) {
     return deriv(p, in, out, delta, element, stream, zm::OP_DY4, true);
+}
+zm_status zm_dz4(zm_plan* p, const void* in, void* out, double delta, zm_element element, void* stream) {
+    return deriv(p, in, out, delta, element, stream, zm::OP_DZ4, false);
+}
+zm_status zm_dz4_adjoint(zm_plan* p, const void* in, void* out, double delta, zm_element element, void* stream) {
+    return deriv(p, in, out, delta, element, stream, zm::OP_DZ4, true);
 }
 
 static zm_status spectral(zm_plan* p, const void* in, void* out, zm_spectral kind, double p0, double p1,

@@ -186,8 +186,11 @@ __global__ void __launch_bounds__(256) laplace_kernel(const E* __restrict__ in, 
 }
 
 // [1, -8, 0, 8, -1] / (12 delta) in the reference's order (dx.cu:29-79):
-// s = -8 f(-1) + f(-2) + 8 f(+1) - f(+2), then s / (12 delta).
-template <typename E, typename S, bool ALONG_X>
+// s = -8 f(-1) + f(-2) + 8 f(+1) - f(+2), then s / (12 delta).  AXIS 0 = x
+// (packed columns), 1 = y (block-cyclic rows), 2 = z (block-cyclic depth, 3D
+// only; the reference has no z derivative).  The other coordinates are kept,
+// so the 2D kernels are the 3D ones on a one-plane slab.
+template <typename E, typename S, int AXIS>
 __global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, S denom,
                                                      int negate) {
 #pragma clang fp contract(off)
@@ -201,12 +204,15 @@ __global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         int a, b2;
-        if (ALONG_X) {
+        if (AXIS == 0) {
             x_walk(g, p.cid, p.lc >> g.lvec, p.lc & (g.vec - 1), steps[k], &a, &b2);
-            f[k] = src[encode(g, 0, p.rid, a, 0, p.lr, b2)];
-        } else {
+            f[k] = src[encode(g, p.did, p.rid, a, p.ld, p.lr, b2)];
+        } else if (AXIS == 1) {
             bc_walk(g, p.rid, p.lr, steps[k], &a, &b2);
-            f[k] = src[encode(g, 0, a, p.cid, 0, b2, p.lc)];
+            f[k] = src[encode(g, p.did, a, p.cid, p.ld, b2, p.lc)];
+        } else {
+            bc_walk(g, p.did, p.ld, steps[k], &a, &b2);
+            f[k] = src[encode(g, a, p.rid, p.cid, b2, p.lr, p.lc)];
         }
     }
     E s;
@@ -383,10 +389,13 @@ static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int 
         hipLaunchKernelGGL((laplace_kernel<E, S>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
                            coef != 1.0 ? 1 : 0);
     } else if (op == OP_DX4) {
-        hipLaunchKernelGGL((deriv4_kernel<E, S, true>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
+        hipLaunchKernelGGL((deriv4_kernel<E, S, 0>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
+                           negate ? 1 : 0);
+    } else if (op == OP_DY4) {
+        hipLaunchKernelGGL((deriv4_kernel<E, S, 1>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
                            negate ? 1 : 0);
     } else {
-        hipLaunchKernelGGL((deriv4_kernel<E, S, false>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
+        hipLaunchKernelGGL((deriv4_kernel<E, S, 2>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
                            negate ? 1 : 0);
     }
     return hipGetLastError();

@@ -252,6 +252,13 @@ class Plan:
         L.check(fn(self._h, _ptr(x), _ptr(out), float(delta), self._elem(x), _stream(stream)), "zm_dy4")
         return out
 
+    def dz4(self, x, delta: float = 1.0, out=None, stream=None, adjoint: bool = False):
+        """4th-order z (depth) derivative; 3D plans only."""
+        out = self._out(x, out, real=not x.is_complex())
+        fn = L._lib().zm_dz4_adjoint if adjoint else L._lib().zm_dz4
+        L.check(fn(self._h, _ptr(x), _ptr(out), float(delta), self._elem(x), _stream(stream)), "zm_dz4")
+        return out
+
     SPECTRAL_KINDS = {"poisson": L.ZM_SPECTRAL_POISSON, "heat": L.ZM_SPECTRAL_HEAT, "scale": L.ZM_SPECTRAL_SCALE,
                       "stencil": L.ZM_SPECTRAL_STENCIL, "poisson_stencil": L.ZM_SPECTRAL_POISSON_STENCIL}
 

@@ -31,7 +31,7 @@ def plan(dims, n, p, b, precision="fp64", batch=1):
     return Plan(dims, n, p, b, precision=precision, batch=batch)
 
 
-@pytest.mark.parametrize("dims,n,p,b", [(2, 16, 2, 2), (3, 8, 2, 1)])
+@pytest.mark.parametrize("dims,n,p,b", [(2, 16, 2, 2), (3, 8, 2, 1), (3, 16, 2, 2)])
 def test_gradcheck_every_operator(dims, n, p, b):
     from stencilfft_amd import autograd as ag
     note(f"gradcheck {dims}D N={n}")
@@ -40,9 +40,12 @@ def test_gradcheck_every_operator(dims, n, p, b):
     ops = [lambda t: ag.fft(t, pl), lambda t: ag.ifft(t, pl, 0.3), lambda t: ag.laplace(t, pl, 2.0),
            lambda t: ag.spectral(t, pl, "heat", 0.01, 0.5), lambda t: ag.spectral(t, pl, "poisson"),
            lambda t: ag.spectral(t, pl, "stencil", 1.5, 0.0), lambda t: ag.spectral(t, pl, "poisson_stencil", 2.0, 0.0)]
+    if b >= 2:
+        ops += [lambda t: ag.dx4(t, pl, 0.5), lambda t: ag.dy4(t, pl, 0.25)]
+    if dims == 3 and b >= 2:
+        ops += [lambda t: ag.dz4(t, pl, 0.75)]
     if dims == 2:
-        ops += [lambda t: ag.dx4(t, pl, 0.5), lambda t: ag.dy4(t, pl, 0.25),
-                lambda t: ag.spectral(t, pl, "stencil", 1.5, 0.5),
+        ops += [lambda t: ag.spectral(t, pl, "stencil", 1.5, 0.5),
                 lambda t: ag.spectral(t, pl, "poisson_stencil", 2.0, 0.25)]
     for op in ops:
         assert torch.autograd.gradcheck(op, (x,), eps=1e-6, atol=1e-8, rtol=1e-6, fast_mode=True)

@@ -258,6 +258,36 @@ def test_dx_dy_fp64_bit_exact(oracle_mod, n, p, b):
             assert np.array_equal(ax, -gx)
 
 
+@pytest.mark.parametrize("n,p,b", [(16, 2, 2), (32, 2, 4), (32, 4, 2), (64, 2, 8), (128, 2, 8)])
+def test_dx_dy_dz_3d_fp64_bit_exact(oracle_mod, n, p, b):
+    """3D derivatives along x, y and z against the oracle's natural-layout
+    restatement (itself bit-identical to dx.cu / dy.cu in 2D)."""
+    note(f"3D dx/dy/dz N={n} P={p} B={b}")
+    s = oracle_mod.Shape(3, n, p, b)
+    rng = np.random.default_rng(510 + n)
+    xr = s.natural_to_real(rng.uniform(-1, 1, (n, n, n)))
+    xc = s.natural_to_real(uniform_field((n, n, n), 511 + n))
+    pl = plan(3, n, p, b, "fp64")
+    fns = {"x": pl.dx4, "y": pl.dy4, "z": pl.dz4}
+    for delta in (1.0, 1.0 / n, 0.3):
+        for x, dt in ((xr, torch.float64), (xc, torch.complex128)):
+            for axis, fn in fns.items():
+                got = host(fn(dev(x, dt), delta))
+                assert np.array_equal(got.view(np.int64), s.deriv4(x, axis, delta).view(np.int64)), axis
+                assert np.array_equal(host(fn(dev(x, dt), delta, adjoint=True)), -got)
+
+
+def test_dz4_needs_3d():
+    from stencilfft_amd import ZmError
+    note("dz4 on a 2D plan")
+    with pytest.raises(ZmError) as e:
+        plan(2, 64, 2, 2, "fp64").dz4(torch.zeros(64 * 64, dtype=torch.float64, device="cuda"))
+    assert e.value.status == 3
+    with pytest.raises(ZmError) as e:
+        plan(3, 16, 2, 1, "fp64").dz4(torch.zeros(16 ** 3, dtype=torch.float64, device="cuda"))
+    assert e.value.status == 3
+
+
 @pytest.mark.parametrize("dims,n,p,b", [(2, 256, 4, 8), (3, 32, 2, 4), (3, 512, 2, 16)])
 def test_stencils_fp32(oracle_mod, dims, n, p, b):
     note(f"fp32 stencils {dims}D N={n}")
@@ -271,6 +301,15 @@ def test_stencils_fp32(oracle_mod, dims, n, p, b):
         got = host(pl.dx4(dev(x, torch.complex64), 1.0 / n)).astype(np.complex128)
         want = s.dx(x.astype(np.complex128), 1.0 / n)
         assert rel_l2(got, want) < 1e-6
+    else:
+        # against the FP64 GPU derivatives (bit-exact to the oracle above) on the same rounded input
+        p64 = plan(dims, n, p, b, "fp64")
+        x32 = dev(x, torch.complex64)
+        x64 = x32.to(torch.complex128)
+        for f32, f64 in ((pl.dx4, p64.dx4), (pl.dy4, p64.dy4), (pl.dz4, p64.dz4)):
+            a = f32(x32, 1.0 / n).to(torch.complex128)
+            r = f64(x64, 1.0 / n)
+            assert (torch.linalg.vector_norm(a - r) / torch.linalg.vector_norm(r)).item() < 1e-6
 
 
 def test_stencil_natural_layout_definitions(oracle_mod):
@@ -310,13 +349,11 @@ def test_adjoint_identities(oracle_mod, dims, n, p, b):
     Lx = host(pl.laplace(dev(x, torch.complex128)))
     Ly = host(pl.laplace(dev(y, torch.complex128)))
     assert abs(_dot(Lx, y) - _dot(x, Ly)) < 1e-10 * abs(_dot(Lx, y))
-    if dims == 2:
-        Dx = host(pl.dx4(dev(x, torch.complex128), 0.25))
-        DHy = host(pl.dx4(dev(y, torch.complex128), 0.25, adjoint=True))
-        assert abs(_dot(Dx, y) - _dot(x, DHy)) < 1e-10 * abs(_dot(Dx, y))
-        Dy = host(pl.dy4(dev(x, torch.complex128), 0.25))
-        DyH = host(pl.dy4(dev(y, torch.complex128), 0.25, adjoint=True))
-        assert abs(_dot(Dy, y) - _dot(x, DyH)) < 1e-10 * abs(_dot(Dy, y))
+    for name in ("dx4", "dy4", "dz4") if dims == 3 else ("dx4", "dy4"):
+        fn = getattr(pl, name)
+        D = host(fn(dev(x, torch.complex128), 0.25))
+        DH = host(fn(dev(y, torch.complex128), 0.25, adjoint=True))
+        assert abs(_dot(D, y) - _dot(x, DH)) < 1e-10 * abs(_dot(D, y)), name
 
 
 @pytest.mark.parametrize("dims,n,p,b", [(2, 64, 4, 2), (3, 32, 2, 4)])

@@ -260,6 +260,61 @@ def test_stencils_plane_wave_eigenvalues(oracle_mod):
         assert np.allclose(dy, 1j * (8 * np.sin(ty) - np.sin(2 * ty)) / (6 * 0.5) * f, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,p,b", [(16, 2, 2), (32, 2, 4), (64, 4, 2), (32, 2, 8)])
+def test_deriv4_natural_equals_reference_dx_dy_bitwise(oracle_mod, n, p, b):
+    """The natural-layout restatement (used for 3D) reproduces the C restatement
+    of dx.cu / dy.cu bit for bit on 2D grids, real and complex."""
+    s = oracle_mod.Shape(2, n, p, b)
+    rng = np.random.default_rng(42 + n)
+    for x in (s.natural_to_real(rng.uniform(-1, 1, (n, n))), s.natural_to_real(uniform_field((n, n), 43 + n))):
+        for delta in (1.0, 1.0 / n, 0.3):
+            assert np.array_equal(s.deriv4(x, "x", delta).view(np.int64), s.dx(x, delta).view(np.int64))
+            assert np.array_equal(s.deriv4(x, "y", delta).view(np.int64), s.dy(x, delta).view(np.int64))
+
+
+@pytest.mark.parametrize("n,p,b", [(16, 2, 2), (32, 2, 4), (32, 4, 2)])
+def test_deriv4_3d_plane_wave_eigenvalues(oracle_mod, n, p, b):
+    """dx, dy, dz of exp(i (tx gc + ty gr + tz gd)) = i (8 sin t - sin 2t) / (6 delta) times the wave."""
+    s = oracle_mod.Shape(3, n, p, b)
+    gd, gr, gc = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    for kx, ky, kz in [(1, 0, 0), (0, 3, 0), (0, 0, 5), (3, 5, 7), (n // 2, 1, n // 2 - 1)]:
+        t = {a: 2 * np.pi * k / n for a, k in (("x", kx), ("y", ky), ("z", kz))}
+        f = np.exp(1j * (t["x"] * gc + t["y"] * gr + t["z"] * gd))
+        g = s.natural_to_real(f)
+        for axis in "xyz":
+            d = s.real_to_natural(s.deriv4(g, axis, 0.5))
+            want = 1j * (8 * np.sin(t[axis]) - np.sin(2 * t[axis])) / (6 * 0.5) * f
+            assert np.max(np.abs(d - want)) < 1e-12, (axis, kx, ky, kz)
+    # a field constant along z has a zero z derivative (to rounding: the
+    # reference's summation order does not cancel exactly) and 2D x / y
+    # derivatives per plane
+    s2 = oracle_mod.Shape(2, n, p, b)
+    plane = uniform_field((n, n), 44)
+    vol = np.broadcast_to(plane, (n, n, n)).copy()
+    g = s.natural_to_real(vol)
+    assert np.max(np.abs(s.deriv4(g, "z"))) < 1e-15
+    dx2 = s2.real_to_natural(s2.dx(s2.natural_to_real(plane)))
+    assert np.array_equal(s.real_to_natural(s.deriv4(g, "x"))[n // 3], dx2)
+    with pytest.raises(ValueError):
+        s2.deriv4(s2.natural_to_real(plane), "z")
+
+
+def test_deriv4_planted_mistakes(oracle_mod):
+    """A wrong sign, a swapped axis or a dropped outer term fails the checks above."""
+    n, p, b = 16, 2, 2
+    s = oracle_mod.Shape(3, n, p, b)
+    gd, gr, gc = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    t = 2 * np.pi * 3 / n
+    f = np.exp(1j * t * gd)
+    want = 1j * (8 * np.sin(t) - np.sin(2 * t)) / 6 * f
+    good = oracle_mod.deriv4_natural(f, 0)
+    assert np.max(np.abs(good - want)) < 1e-12
+    assert np.max(np.abs(-good - want)) > 1e-3                                   # sign
+    assert np.max(np.abs(oracle_mod.deriv4_natural(f, 2) - want)) > 1e-3        # axis
+    two_point = (np.roll(f, -1, 0) - np.roll(f, 1, 0)) * 8 / 12                   # dropped +-2 terms
+    assert np.max(np.abs(two_point - want)) > 1e-3
+
+
 def test_dx_dy_reject_b1(oracle_mod):
     s = oracle_mod.Shape(2, 64, 2, 1)
     with pytest.raises(ValueError):
