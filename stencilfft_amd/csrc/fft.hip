@@ -466,9 +466,12 @@ hipError_t launch_stage_b(const Cx<T>* in, Cx<T>* out, const Geo& g, const Stage
 template <typename T>
 hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const RepackArgs& a, int batch, bool fwd,
                          hipStream_t stream) {
-    // tiled path: VEC >= 16 and L >= 32 (x tile 32 or 16, y tile 64 or 32)
+    // tiled path: VEC >= 16 and L >= 32 (x tile 32 or 16; y tile 128 for
+    // FP32 when L >= 128, else 64 or 32: the largest measured fastest, and a
+    // 32 x 128 tile of complex128 would not fit static LDS)
     if (g.vec >= 16 && g.l >= 32) {
-        const int ltx = g.vec >= 32 ? 5 : 4, lty = g.l >= 64 ? 6 : 5;
+        const int ltx = g.vec >= 32 ? 5 : 4;
+        const int lty = (sizeof(T) == 4 && g.l >= 128) ? 7 : (g.l >= 64 ? 6 : 5);
         int lecount = 0;
         while ((1 << lecount) < a.ecount) lecount++;
         const unsigned blocks =
@@ -479,6 +482,9 @@ hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const Repack
         hipLaunchKernelGGL((repack_tiled_kernel<T, F, X, Y>), grid, dim3(256), 0, stream, in, out, g, a, lecount); \
         return hipGetLastError();                                                                              \
     }
+        if constexpr (sizeof(T) == 4) {
+            ZM_REPACK(true, 5, 7) ZM_REPACK(true, 4, 7) ZM_REPACK(false, 5, 7) ZM_REPACK(false, 4, 7)
+        }
         ZM_REPACK(true, 5, 6) ZM_REPACK(true, 5, 5) ZM_REPACK(true, 4, 6) ZM_REPACK(true, 4, 5)
         ZM_REPACK(false, 5, 6) ZM_REPACK(false, 5, 5) ZM_REPACK(false, 4, 6) ZM_REPACK(false, 4, 5)
 #undef ZM_REPACK

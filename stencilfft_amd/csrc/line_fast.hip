@@ -15,6 +15,7 @@
 //   * the last radix pass writes straight to HBM with lanes running along the
 //     contiguous output index.
 #include "common.hpp"
+#include <type_traits>
 #include "kernels.hpp"
 
 namespace zm {
@@ -321,6 +322,10 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
     if constexpr (!TOGLOBAL) __syncthreads();
 }
 
+struct NoHook {
+    template <typename X>
+    __device__ void operator()(X) const {}
+};
 struct NoSink {
     template <typename C>
     __device__ void operator()(int, int, C) const {}
@@ -331,9 +336,9 @@ struct NoSink {
 // OOP the passes alternate between the buffers `cur` and `alt`; the return
 // value is the buffer that holds the result.
 template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool FULLTAB, bool MF, bool OOP>
+          bool FULLTAB, bool MF, bool OOP, int HOFF = 0, typename Hook = NoHook>
 __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const Cx<T>* ts, int ls, const Cx<T>* ta,
-                                                  const Cx<T>* tb, int tid, const Mfma16W* w) {
+                                                  const Cx<T>* tb, int tid, const Mfma16W* w, Hook&& hook = Hook{}) {
     if constexpr (PASS < NPASS) {
         constexpr int R = fast::Radix<LN_T, PASS>::value;
         constexpr int NS = 1 << (4 * PASS);
@@ -344,8 +349,9 @@ __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const 
         else
             fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
                 cur, dst, ts, ls, ta, tb, tid, NoSink{});
-        return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP>(
-            dst, OOP ? cur : alt, ts, ls, ta, tb, tid, w);
+        hook(std::integral_constant<int, HOFF + PASS + 1>{});
+        return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP, HOFF>(
+            dst, OOP ? cur : alt, ts, ls, ta, tb, tid, w, hook);
     } else {
         return cur;
     }
@@ -448,6 +454,16 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
 
     // register prefetch of one line group: element e = tid + k*NT
     C pre[EPT];
+    auto fetch_part = [&](auto k0c, auto k1c, int grp) {
+        constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
+        const C* gb = src0 + group_base(grp);
+#pragma unroll
+        for (int k = K0; k < K1; k++) {
+            const int e = tid + k * NT;
+            const int l = e >> LN, rem = e & (N - 1);
+            pre[k] = ldg(gb + off(rem >> ll, l, rem & (L - 1)));
+        }
+    };
     auto fetch = [&](int grp) {
         const C* gb = src0 + group_base(grp);
 #pragma unroll
@@ -505,19 +521,37 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         stage();
         __syncthreads();
         const int nxt = grp + gridDim.x;
-        if (nxt < groups) fetch(nxt);
+        // FP32: the next group's loads are spread over the pass boundaries
+        // (HN points: after staging and after every LDS pass before the
+        // last), a burst of EPT loads per thread stalls instruction issue
+        constexpr bool SPREAD = sizeof(T) == 4 && !MF;
+        constexpr int NP1 = FWD ? NPV : NPM;
+        constexpr int NP2 = STAGED ? (FWD ? NPM : NPV) : ((FWD ? NPM : NPV) - 1);
+        constexpr int HN = 1 + NP1 + NP2;
+        auto hook = [&](auto hc) {
+            constexpr int h = decltype(hc)::value;
+            if constexpr (SPREAD && h < HN) {
+                constexpr int K0 = h * EPT / HN, K1 = (h + 1) * EPT / HN;
+                if constexpr (K1 > K0) {
+                    if (nxt < groups)
+                        fetch_part(std::integral_constant<int, K0>{}, std::integral_constant<int, K1>{}, nxt);
+                }
+            }
+        };
+        if constexpr (SPREAD) hook(std::integral_constant<int, 0>{});
+        else if (nxt < groups) fetch(nxt);
 
         C* const gd = dst0 + group_base(grp);
 
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
-            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP>(
-                lds, lds2, ts, LS, ta, tb, tid, w);
+            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0>(
+                lds, lds2, ts, LS, ta, tb, tid, w, hook);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
             if constexpr (STAGED) {
-                cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP, NP1>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
                 // element k2 of transform (line, k1) is at line*N + k1 + k2*VEC;
                 // sweep each destination row (rank gq, position fr + k2*VP)
 #pragma unroll
@@ -531,8 +565,8 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                     stg(gd + off(gq, l, pos), v);
                 }
             } else {
-                cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP, NP1>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
                 constexpr int PL = NPM - 1;
                 constexpr int RL = fast::Radix<LM, PL>::value;
                 constexpr int NSL = 1 << (4 * PL);
@@ -550,12 +584,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         } else {
             // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
             // the conjugate inter-stage twiddle; the last pass stores to HBM
-            C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP>(
-                lds, lds2, ts, LS, ta, tb, tid, w);
+            C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP, 0>(
+                lds, lds2, ts, LS, ta, tb, tid, w, hook);
             const T sc = MF ? scale * inv_s0 : scale;
             if constexpr (STAGED) {
-                cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP, NP1>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
                 // element jj of transform (line, r = gm*B + j) is at
                 // line*N + r*VEC + jj = line*N + gm*L + (j*VEC + jj): the
                 // group's LDS order is the destination rows' order
@@ -568,8 +602,8 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                     stg(gd + off(rem >> ll, l, rem & (L - 1)), v);
                 }
             } else {
-                cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w);
+                cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP, NP1>(
+                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
                 constexpr int PL = NPV - 1;
                 constexpr int RL = fast::Radix<LVEC, PL>::value;
                 constexpr int NSL = 1 << (4 * PL);
