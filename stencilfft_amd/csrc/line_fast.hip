@@ -79,6 +79,29 @@ __device__ __forceinline__ C tw_n(const C* ta, const C* tb, int e) {
     return ctw<FWD>(C{1, 0}, cmul(a, b));
 }
 
+// Per-thread PRE twiddle table (see fast_pass): vector-ALU path, full W_N
+// table in LDS and one line per workgroup, where the per-thread table has the
+// same N entries as W_N.
+template <bool MF, bool FULLTAB, int LN, int LE>
+constexpr bool kPerThreadPre = !MF && FULLTAB && LE == LN;
+
+// Exponent e of the PRE twiddle W_N^e of element r of work item u in the
+// first pass of a stage (the index arithmetic of fast_pass).
+template <int LN_T, int R, bool CONTIG, int LVEC, int LN>
+__device__ __forceinline__ int pre_exp(int u, int r) {
+    constexpr int n = 1 << LN_T, nbf = n / R, LNBF = fast::Log2<nbf>::value;
+    constexpr int N = 1 << LN, VEC = 1 << LVEC;
+    int j, kk;
+    if (CONTIG) {
+        j = u & (nbf - 1);
+        kk = (u >> LNBF) & ((N >> LN_T) - 1);
+    } else {
+        j = (u >> LVEC) & (nbf - 1);
+        kk = u & (VEC - 1);
+    }
+    return ((j + r * nbf) * kk) & (N - 1);
+}
+
 // One Stockham radix-R pass over the line-group held in LDS.
 //   CONTIG: transforms are contiguous runs of length n (the VEC-point DFTs,
 //           batch index bt = (line, r)); lanes run along j.
@@ -89,8 +112,13 @@ __device__ __forceinline__ C tw_n(const C* ta, const C* tb, int e) {
 // OOP: read `src`, write `dst` (a different LDS buffer) as soon as each
 // transform is done, one barrier at the end; otherwise src == dst and all
 // reads finish (barrier) before the writes start.
+// Twiddle tables (LDS): PTW: `ta` holds this thread's PRE twiddles at
+// (k*R + r)*NT + tid (built by the kernel with pre_exp); otherwise the
+// whole W_N (FULLTAB) or the two-level A/B pair.  PPW: `ts` is this stage's
+// per-pass table, W_{NS R}^(jm r) at r*NS + jm (jm runs along the lanes, so
+// both PTW and PPW reads are free of bank conflicts); otherwise `ts` is W_S.
 template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool TOGLOBAL, bool FULLTAB, bool OOP, typename Sink>
+          bool TOGLOBAL, bool FULLTAB, bool OOP, bool PTW, bool PPW, typename Sink>
 __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, int ls, const Cx<T>* ta,
                                           const Cx<T>* tb, int tid, Sink&& sink) {
     using C = Cx<T>;
@@ -103,6 +131,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
     constexpr int BPT = EPT / R;
     constexpr int ES = CONTIG ? 1 : VEC;
     static_assert(BPT * NT * R == E, "work split");
+    static_assert(!PPW || NS == 1 || NS == 16, "per-pass tables cover the second pass of a stage");
     C v[BPT][R];
     int obase[BPT];
 #pragma unroll
@@ -129,16 +158,22 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
             for (int r = 0; r < R; r++) {
                 const int e = ((j + r * nbf) * kk) & (N - 1);
                 // FULLTAB: ta is the whole W_N table; otherwise W_N^e = A[e & 63] * B[e >> 6]
-                if constexpr (FULLTAB) v[k][r] = ctw<FWD>(v[k][r], ta[e]);
+                if constexpr (PTW) v[k][r] = ctw<FWD>(v[k][r], ta[(k * R + r) * NT + tid]);
+                else if constexpr (FULLTAB) v[k][r] = ctw<FWD>(v[k][r], ta[e]);
                 else v[k][r] = ctw<FWD>(v[k][r], cmul(ta[e & 63], tb[e >> 6]));
             }
         }
         if (NS > 1) {
             const int jm = j & (NS - 1);
-            // W_{NS*R}^(jm*r) = W_S^(jm*r*S/(NS*R)), S = 2^ls
-            const int step = (1 << ls) / (NS * R);
+            if constexpr (PPW) {
 #pragma unroll
-            for (int r = 1; r < R; r++) v[k][r] = ctw<FWD>(v[k][r], ts[jm * r * step]);
+                for (int r = 1; r < R; r++) v[k][r] = ctw<FWD>(v[k][r], ts[r * NS + jm]);
+            } else {
+                // W_{NS*R}^(jm*r) = W_S^(jm*r*S/(NS*R)), S = 2^ls
+                const int step = (1 << ls) / (NS * R);
+#pragma unroll
+                for (int r = 1; r < R; r++) v[k][r] = ctw<FWD>(v[k][r], ts[jm * r * step]);
+            }
         }
         dft<R, FWD>(v[k]);
         const int jm = j & (NS - 1);
@@ -331,7 +366,6 @@ struct NoSink {
     __device__ void operator()(int, int, C) const {}
 };
 
-// Run passes PASS..NPASS-1 of a length-2^LN_T transform (all in LDS).
 // Run passes PASS..NPASS-1 of a length-2^LN_T transform, all in LDS.  With
 // OOP the passes alternate between the buffers `cur` and `alt`; the return
 // value is the buffer that holds the result.
@@ -347,8 +381,8 @@ __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const 
             mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
                 cur, dst, ts, ls, ta, tb, tid, w, NoSink{});
         else
-            fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
-                cur, dst, ts, ls, ta, tb, tid, NoSink{});
+            fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP,
+                      kPerThreadPre<MF, FULLTAB, LN, LE>, !MF>(cur, dst, ts, ls, ta, tb, tid, NoSink{});
         hook(std::integral_constant<int, HOFF + PASS + 1>{});
         return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP, HOFF>(
             dst, OOP ? cur : alt, ts, ls, ta, tb, tid, w, hook);
@@ -366,8 +400,8 @@ __device__ __forceinline__ void last_pass(const Cx<T>* cur, const Cx<T>* ts, int
         mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false>(cur, nullptr, ts, ls, ta,
                                                                                            tb, tid, w, sink);
     else
-        fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false>(cur, nullptr, ts, ls, ta,
-                                                                                            tb, tid, sink);
+        fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false,
+                  kPerThreadPre<MF, FULLTAB, LN, LE>, !MF>(cur, nullptr, ts, ls, ta, tb, tid, sink);
 }
 
 // Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
@@ -397,18 +431,45 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     const int tid = threadIdx.x;
 
     // global table layout: [W_S | A (64) | B (N/64) | W_N (N)]
-    if constexpr (FULLTAB) {
-        for (int t = tid; t < S; t += NT) tabs[t] = tables[t];
-        for (int t = tid; t < N; t += NT) tabs[S + t] = tables[NTWO + t];
+    // LDS table layout:    [W_S | per-pass V (VEC) | per-pass M (M) | PRE]
+    // PRE: per-thread table (kPerThreadPre, E = N entries), else W_N (FULLTAB),
+    // else A | B.  The per-pass tables hold W_{16 R}^(jm r) at r*16 + jm for
+    // the second pass (NS = 16, radix R) of the VEC and M stages.
+    constexpr bool PTW = kPerThreadPre<MF, FULLTAB, LN, LE>;
+    constexpr int TV = S, TM = S + VEC, TP = S + VEC + M;
+    for (int t = tid; t < S; t += NT) tabs[t] = tables[t];
+    if constexpr (NPV > 1) {
+        constexpr int RV = fast::Radix<LVEC, 1>::value;
+        for (int t = tid; t < 16 * RV; t += NT) tabs[TV + t] = tables[(t & 15) * (t >> 4) * (S / (16 * RV))];
+    }
+    if constexpr (NPM > 1) {
+        constexpr int RM = fast::Radix<LM, 1>::value;
+        for (int t = tid; t < 16 * RM; t += NT) tabs[TM + t] = tables[(t & 15) * (t >> 4) * (S / (16 * RM))];
+    }
+    if constexpr (PTW) {
+        // the PRE pass is the first M pass (forward) or the first VEC pass (inverse)
+        constexpr int LN_P = FWD ? LM : LVEC;
+        constexpr int RP = fast::Radix<LN_P, 0>::value;
+#pragma unroll 1
+        for (int k = 0; k < EPT / RP; k++)
+#pragma unroll 1
+            for (int r = 0; r < RP; r++)
+                tabs[TP + (k * RP + r) * NT + tid] =
+                    tables[NTWO + pre_exp<LN_P, RP, !FWD, LVEC, LN>(tid + k * NT, r)];
+    } else if constexpr (FULLTAB) {
+        for (int t = tid; t < N; t += NT) tabs[TP + t] = tables[NTWO + t];
     } else {
-        for (int t = tid; t < NTWO; t += NT) tabs[t] = tables[t];
+        for (int t = tid; t < NTWO - S; t += NT) tabs[TP + t] = tables[S + t];
     }
     const C* ts = tabs;
-    const C* ta = tabs + S;
-    const C* tb = tabs + S + 64;
+    const C* ta = tabs + TP;
+    const C* tb = tabs + TP + 64;
+    // per-pass tables of the vector-ALU passes; the MFMA path indexes W_S
+    const C* tsv = MF ? ts : tabs + TV;
+    const C* tsm = MF ? ts : tabs + TM;
     // MF: the per-lane MFMA weights and two slots of per-wave maxima follow
     // the tables in LDS
-    Mfma16W* w = reinterpret_cast<Mfma16W*>(tabs + (FULLTAB ? S + N : NTWO));
+    Mfma16W* w = reinterpret_cast<Mfma16W*>(tabs + TP + (FULLTAB ? N : NTWO - S));
     float* wmax = reinterpret_cast<float*>(w + 64);
     constexpr int NW = NT / 64;
     static_assert(!MF || NW <= 16, "maxima slots");
@@ -546,12 +607,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
             C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0>(
-                lds, lds2, ts, LS, ta, tb, tid, w, hook);
+                lds, lds2, tsv, LS, ta, tb, tid, w, hook);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
             if constexpr (STAGED) {
                 cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsm, LS, ta, tb, tid, w, hook);
                 // element k2 of transform (line, k1) is at line*N + k1 + k2*VEC;
                 // sweep each destination row (rank gq, position fr + k2*VP)
 #pragma unroll
@@ -566,12 +627,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 }
             } else {
                 cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsm, LS, ta, tb, tid, w, hook);
                 constexpr int PL = NPM - 1;
                 constexpr int RL = fast::Radix<LM, PL>::value;
                 constexpr int NSL = 1 << (4 * PL);
                 last_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, FULLTAB, MF>(
-                    cur, ts, LS, ta, tb, tid, w, [&](int u, int k2, C v) {
+                    cur, tsm, LS, ta, tb, tid, w, [&](int u, int k2, C v) {
                         // transform (line, k1): k1 = u & (VEC-1), line = u >> (LVEC + log2(nbf))
                         constexpr int nbf = (1 << LM) / RL;
                         const int k1 = u & (VEC - 1);
@@ -585,11 +646,11 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
             // the conjugate inter-stage twiddle; the last pass stores to HBM
             C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP, 0>(
-                lds, lds2, ts, LS, ta, tb, tid, w, hook);
+                lds, lds2, tsm, LS, ta, tb, tid, w, hook);
             const T sc = MF ? scale * inv_s0 : scale;
             if constexpr (STAGED) {
                 cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsv, LS, ta, tb, tid, w, hook);
                 // element jj of transform (line, r = gm*B + j) is at
                 // line*N + r*VEC + jj = line*N + gm*L + (j*VEC + jj): the
                 // group's LDS order is the destination rows' order
@@ -603,12 +664,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 }
             } else {
                 cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, ts, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsv, LS, ta, tb, tid, w, hook);
                 constexpr int PL = NPV - 1;
                 constexpr int RL = fast::Radix<LVEC, PL>::value;
                 constexpr int NSL = 1 << (4 * PL);
                 last_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, FULLTAB, MF>(
-                    cur, ts, LS, ta, tb, tid, w, [&](int u, int jj, C v) {
+                    cur, tsv, LS, ta, tb, tid, w, [&](int u, int jj, C v) {
                         // transform (line, r): r = bt & (M-1), line = bt >> LM
                         constexpr int nbf = (1 << LVEC) / RL;
                         const int bt = u >> fast::Log2<nbf>::value;
@@ -642,7 +703,9 @@ int lines_per_iter_log2(int n) {
 size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false) {
     const int n = vec * m, s = vec > m ? vec : m;
     const int e = n << lines_per_iter_log2(n);
-    const int tab = fulltab ? s + n : s + 64 + (n / 64 > 0 ? n / 64 : 1);
+    // W_S, the two per-pass tables, then W_N (or the per-thread PRE table of
+    // the same size) or the two-level pair
+    const int tab = s + vec + m + (fulltab ? n : 64 + (n / 64 > 0 ? n / 64 : 1));
     const int shift = n >= 4096 ? (es == 16 ? 6 : 4) : 5;
     const int padded = e + (e >> shift) + 1;
     return es * (padded * (mf ? 2 : 1) + tab) + (mf ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0);
