@@ -308,11 +308,9 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
     constexpr int XA = TY / V, XR = 256 / XA;  // x/y side: accesses per row, rows per sweep
     constexpr int PA = TX / V, PR = 256 / PA;  // packed side
     static_assert(XR >= 1 && TX % XR == 0 && PR >= 1 && TY % PR == 0, "tile split");
-    // non-temporal for FP64, plain for FP32 (whichever measured faster)
-    auto ldv = [](const Vt* p) -> Vt {
-        if constexpr (sizeof(T) == 8) return ld_nt(p);
-        else return *p;
-    };
+    // non-temporal loads; non-temporal stores for FP64 only (FP32 plain
+    // stores measured faster over whole steps, line pass included)
+    auto ldv = [](const Vt* p) -> Vt { return ld_nt(p); };
     auto stv = [](Vt* p, Vt v) {
         if constexpr (sizeof(T) == 8) st_nt(p, v);
         else *p = v;
