@@ -17,6 +17,7 @@
 // that stays accurate on smooth fields (see lap_combine).
 #include "common.hpp"
 #include "kernels.hpp"
+#include <type_traits>
 
 namespace zm {
 
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E
 // lds[pos + 2 VEC].  3D: the rows of planes gd - 1 and gd + 1 are loaded
 // directly; the strips of those planes are grid.x * grid.y (a multiple of 8)
 // blocks away, so they run on the same XCD and the loads hit its L2.
-template <typename E, typename S, int K, bool D3>
+template <typename E, typename S, int K, bool D3, int HK>
 __global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g,
                                                             int rows_per_strip, S scale, int apply_scale) {
 #pragma clang fp contract(off)
@@ -276,60 +277,98 @@ __global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict_
         const int id = (((did << g.lp) + rid) << g.lp) + c;
         return (int64_t)id * g.local + ((int64_t)ldp * L + lr) * L;
     };
-    E up[K], ce[K], dn[K];
-    {
-        const int64_t ou = row_off(gd, g0 - 1, cid), oc = row_off(gd, g0, cid);
+    // One row of lookahead: while row gr is computed, the loads of row gr + 2
+    // (the next row's row below), of row gr + 1's halo vectors and, in 3D, of
+    // row gr + 1's z-neighbours are in flight.  The rows live in a ring of
+    // four register slots and the loop is unrolled by four (strips hold a
+    // multiple of 4 rows), so no register copy forces an early wait.
+    // Halo vector element jj = tid + 256 k, k < HK (VEC <= 256 HK <= L = 256 K).
+    auto load_halo = [&](int gr, E* hl, E* hr) __attribute__((always_inline)) {
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            up[k] = ld(src + ou + tid + 256 * k);
-            ce[k] = ld(src + oc + tid + 256 * k);
-        }
-    }
-    for (int t = 0; t < rows_per_strip; t++) {
-        const int gr = g0 + t;
-        const int64_t od = row_off(gd, gr + 1, cid);
-#pragma unroll
-        for (int k = 0; k < K; k++) dn[k] = ld(src + od + tid + 256 * k);
-        E fr[K], bk[K];
-        if constexpr (D3) {
-            const int64_t of = row_off(gd - 1, gr, cid), ob = row_off(gd + 1, gr, cid);
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                fr[k] = ld(src + of + tid + 256 * k);
-                bk[k] = ld(src + ob + tid + 256 * k);
+        for (int k = 0; k < HK; k++) {
+            const int jj = tid + 256 * k;
+            if (jj < VEC) {
+                const int jl = (cid == 0) ? ((jj - 1 + VEC) & (VEC - 1)) : jj;
+                const int jr = (cid == P - 1) ? ((jj + 1) & (VEC - 1)) : jj;
+                hl[k] = src[row_off(gd, gr, cl) + (B - 1) * VEC + jl];
+                hr[k] = src[row_off(gd, gr, cr) + jr];
             }
+        }
+    };
+    auto load_row = [&](int z, int gr, E* r) __attribute__((always_inline)) {
+        const int64_t o = row_off(z, gr, cid);
+#pragma unroll
+        for (int k = 0; k < K; k++) r[k] = ld(src + o + tid + 256 * k);
+    };
+    E rr[4][K], zf[2][K], zb[2][K], hl[2][HK], hr[2][HK];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        rr[3][k] = O::zero();
+        zf[0][k] = zf[1][k] = zb[0][k] = zb[1][k] = O::zero();
+    }
+#pragma unroll
+    for (int k = 0; k < HK; k++) hl[0][k] = hl[1][k] = hr[0][k] = hr[1][k] = O::zero();
+    load_row(gd, g0 - 1, rr[0]);
+    load_row(gd, g0, rr[1]);
+    load_row(gd, g0 + 1, rr[2]);
+    if constexpr (D3) {
+        load_row(gd - 1, g0, zf[0]);
+        load_row(gd + 1, g0, zb[0]);
+    }
+    load_halo(g0, hl[0], hr[0]);
+    // row t of the strip: up / centre / down in slots t, t+1, t+2 (mod 4), the
+    // lookahead row in slot t+3; z-neighbours and halos in slot t (mod 2)
+    auto step = [&](int t, auto uc, auto hc) __attribute__((always_inline)) {
+        constexpr int U = decltype(uc)::value, C0 = (U + 1) & 3, D = (U + 2) & 3, X = (U + 3) & 3;
+        constexpr int H = decltype(hc)::value, HN = H ^ 1;
+        const int gr = g0 + t;
+        if (t + 1 < rows_per_strip) {
+            load_row(gd, gr + 2, rr[X]);
+            if constexpr (D3) {
+                load_row(gd - 1, gr + 1, zf[HN]);
+                load_row(gd + 1, gr + 1, zb[HN]);
+            }
+            load_halo(gr + 1, hl[HN], hr[HN]);
         }
         // stage the current row and its halos
 #pragma unroll
-        for (int k = 0; k < K; k++) row[VEC + tid + 256 * k] = ce[k];
-        for (int jj = tid; jj < VEC; jj += 256) {
-            const int jl = (cid == 0) ? ((jj - 1 + VEC) & (VEC - 1)) : jj;
-            const int jr = (cid == P - 1) ? ((jj + 1) & (VEC - 1)) : jj;
-            row[jj] = src[row_off(gd, gr, cl) + (B - 1) * VEC + jl];
-            row[VEC + L + jj] = src[row_off(gd, gr, cr) + jr];
+        for (int k = 0; k < K; k++) row[VEC + tid + 256 * k] = rr[C0][k];
+#pragma unroll
+        for (int k = 0; k < HK; k++) {
+            const int jj = tid + 256 * k;
+            if (jj < VEC) {
+                row[jj] = hl[H][k];
+                row[VEC + L + jj] = hr[H][k];
+            }
         }
         __syncthreads();
         const int64_t oc = row_off(gd, gr, cid);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int pos = tid + 256 * k;
+            const E c = rr[C0][k];
             E acc;
             if constexpr (D3)
-                acc = lap_combine<sizeof(S) == 8, O, E, S>(up[k], dn[k], row[pos], row[pos + 2 * VEC], fr[k], bk[k],
-                                                           ce[k], true);
+                acc = lap_combine<sizeof(S) == 8, O, E, S>(rr[U][k], rr[D][k], row[pos], row[pos + 2 * VEC], zf[H][k],
+                                                           zb[H][k], c, true);
             else
-                acc = lap_combine<sizeof(S) == 8, O, E, S>(up[k], dn[k], row[pos], row[pos + 2 * VEC], ce[k], ce[k],
-                                                           ce[k], false);
+                acc = lap_combine<sizeof(S) == 8, O, E, S>(rr[U][k], rr[D][k], row[pos], row[pos + 2 * VEC], c, c, c,
+                                                           false);
             if (apply_scale) acc = O::mul(acc, scale);
             if constexpr (D3) dst[oc + pos] = acc;
             else st_nt(dst + oc + pos, acc);
         }
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            up[k] = ce[k];
-            ce[k] = dn[k];
-        }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    for (int t = 0; t < rows_per_strip; t += 4) {
+        step(t, I0{}, I0{});
+        step(t + 1, I1{}, I1{});
+        step(t + 2, I2{}, I0{});
+        step(t + 3, I3{}, I1{});
     }
 }
 
@@ -350,13 +389,21 @@ static hipError_t laplace_strip(const E* in, E* out, const Geo& g, int batch, do
     const size_t lds = sizeof(E) * (size_t)(g.l + 2 * g.vec);
     const S sc = (S)coef;
     const int apply = coef != 1.0 ? 1 : 0;
-#define ZM_STRIP(KK)                                                                                            \
+    // halo vectors: one slot per thread when VEC <= 256, else K
+    const bool one_halo = g.vec <= 256;
+#define ZM_STRIP2(KK, HH)                                                                                        \
     if (d3)                                                                                                     \
-        hipLaunchKernelGGL((laplace_strip_kernel<E, S, KK, true>), grid, dim3(256), lds, stream, in, out, g, rows, \
-                           sc, apply);                                                                          \
+        hipLaunchKernelGGL((laplace_strip_kernel<E, S, KK, true, HH>), grid, dim3(256), lds, stream, in, out, g,  \
+                           rows, sc, apply);                                                                    \
     else                                                                                                        \
-        hipLaunchKernelGGL((laplace_strip_kernel<E, S, KK, false>), grid, dim3(256), lds, stream, in, out, g, rows, \
-                           sc, apply);
+        hipLaunchKernelGGL((laplace_strip_kernel<E, S, KK, false, HH>), grid, dim3(256), lds, stream, in, out, g, \
+                           rows, sc, apply);
+#define ZM_STRIP(KK)                \
+    if (one_halo) {                 \
+        ZM_STRIP2(KK, 1)            \
+    } else {                        \
+        ZM_STRIP2(KK, KK)           \
+    }
     switch (g.l) {
         case 256: ZM_STRIP(1) break;
         case 512: ZM_STRIP(2) break;
@@ -364,12 +411,14 @@ static hipError_t laplace_strip(const E* in, E* out, const Geo& g, int batch, do
         case 2048: ZM_STRIP(8) break;
         default: return hipErrorInvalidValue;
     }
+#undef ZM_STRIP2
 #undef ZM_STRIP
     return hipGetLastError();
 }
 
 static bool strip_ok(const Geo& g, int batch) {
-    const bool lok = g.l == 256 || g.l == 512 || g.l == 1024 || g.l == 2048;
+    // the strip loop is unrolled by four rows
+    const bool lok = (g.l == 256 || g.l == 512 || g.l == 1024 || g.l == 2048) && strip_rows(g, batch) % 4 == 0;
     if (g.dims == 2) return lok;
     // 3D: grid.z = batch * N planes, and grid.x * grid.y a multiple of the 8 XCDs
     const int strips = g.n / strip_rows(g, batch);
