@@ -60,25 +60,6 @@ __host__ __device__ constexpr int fast_padded(int e) {
     return e + (e >> kPadShift<T, LN>) + 1;
 }
 
-
-// Twiddle tables for a line of length N = VEC*M (S = max(VEC, M)):
-//   [0, S)          W_S^t
-//   [S, S+64)       W_N^t,      t < 64
-//   [S+64, S+64+N/64) W_N^(64 t)
-template <typename T>
-struct TwTables {
-    const Cx<T>* ts;
-    const Cx<T>* ta;
-    const Cx<T>* tb;
-};
-
-template <bool FWD, typename C>
-__device__ __forceinline__ C tw_n(const C* ta, const C* tb, int e) {
-    // W_N^e for 0 <= e < N from the two-level table
-    const C a = ta[e & 63], b = tb[e >> 6];
-    return ctw<FWD>(C{1, 0}, cmul(a, b));
-}
-
 // Per-thread PRE twiddle table (see fast_pass): vector-ALU path, full W_N
 // table in LDS and one line per workgroup, where the per-thread table has the
 // same N entries as W_N.
