@@ -19,6 +19,7 @@
 #include <string>
 
 #include "stencilfft.h"
+#include "stencilfft_hdf5.h"
 
 namespace zmodel {
 
@@ -183,6 +184,18 @@ inline void init_host(const zm_shape& s, Complex* host_grid) {
 }
 
 inline void init_host(const zm_shape& s, Scalar* host_grid) { check(zm_init_host(&s, host_grid, 1), "init_host"); }
+
+// init_host_hdf5 / store_host_hdf5 and their _3d forms (zmodel/utils.cu:103-281)
+// for the whole virtual grid, in the reference's file format (one N^d dataset
+// of {"real", "imag"} doubles, natural order).  Callers link
+// libstencilfft_hdf5 as well; nothing else in this header needs it.
+inline void init_host_hdf5(const zm_shape& s, Complex* host_grid, const char* filename, const char* dataset_name) {
+    check(zm_load_hdf5(&s, reinterpret_cast<double*>(host_grid), filename, dataset_name), "init_host_hdf5");
+}
+inline void store_host_hdf5(const zm_shape& s, const Complex* host_grid, const char* filename,
+                            const char* dataset_name) {
+    check(zm_store_hdf5(&s, reinterpret_cast<const double*>(host_grid), filename, dataset_name), "store_host_hdf5");
+}
 
 inline int packed_to_block_cyclic(const zm_shape& s, int c) { return zm_packed_to_block_cyclic(&s, c); }
 inline int block_cyclic_to_packed(const zm_shape& s, int c) { return zm_block_cyclic_to_packed(&s, c); }

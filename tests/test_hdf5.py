@@ -201,3 +201,23 @@ def test_errors(h5io, h5, tmp_path):
     assert e.value.status == ZM_ERR_IO
     with pytest.raises(ValueError):
         h5io.store_host_hdf5((2, 16, 2, 2), np.zeros(100, np.complex128), fn, "data")
+
+
+def test_cpp_facade_wrappers(h5io, tmp_path):
+    """zmodel::store_host_hdf5 / init_host_hdf5 (include/zmodel.hpp) from C++,
+    host code only: compiled with g++ against the two libraries."""
+    import shutil
+    import subprocess
+    from conftest import ROOT
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    pkg = os.path.join(ROOT, "stencilfft_amd")
+    exe = str(tmp_path / "facade_hdf5")
+    cmd = [gxx, "-std=c++17", "-O1", os.path.join(ROOT, "tests", "facade_hdf5.cpp"), "-I", os.path.join(ROOT, "include"),
+           "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-L", pkg, "-lstencilfft_hdf5", "-lstencilfft",
+           f"-Wl,-rpath,{pkg}", "-o", exe]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-2000:]
+    run = subprocess.run([exe, str(tmp_path / "ramp.h5")], capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0 and "facade hdf5 ok" in run.stdout, (run.returncode, run.stdout, run.stderr)
