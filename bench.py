@@ -389,11 +389,20 @@ def main():
             errs[name] = rel_errors(a, r, torch)
         # IFFT(FFT(lap)) against the FP64 Laplacian it started from
         errs["round_trip_vs_fp64_laplace"] = rel_errors(outs[2], outs64[0], torch)
+        del outs64
+        # the 4th-order derivatives (delta = 1/N) on the same rounded input,
+        # outside the timed region
+        derivs = ("dx4", "dy4") if dims == 2 else ("dx4", "dy4", "dz4")
+        if plan.shape.b >= 2:
+            for name in derivs:
+                d32 = getattr(plan, name)(x, 1.0 / n)
+                d64 = getattr(plan64, name)(x64r, 1.0 / n)
+                errs[name] = rel_errors(d32, d64, torch)
+                del d32, d64
         result_extra["errors_vs_fp64_gpu"] = errs
         result_extra["fp64_ms_per_step"] = round(ms64, 4)
         result_extra["speedup_vs_fp64_gpu"] = round(ms64 / ms, 3)
         log(f"errors vs FP64 GPU path: {json.dumps(errs)}")
-        del outs64
         # the FP64 path with the reference's stage structure: per direction
         # FFT1a + pack + exchange, then unpack + twiddle + FFT1b, materialising
         # the post-exchange buffer (zmodel/transforms/fft.cu:199-339)
