@@ -9,9 +9,11 @@
 //     shifts and masks and LDS addresses fold into immediate offsets;
 //   * a workgroup loops over lines; while it transforms line i in LDS the
 //     global loads of line i+1 are already in flight into registers;
-//   * twiddles come from LDS tables (W_S for the sub-transforms, and a
-//     two-level W_N = A[e & 63] * B[e >> 6] for the inter-stage twiddle), all
-//     generated in FP64 on the GPU;
+//   * twiddles come from LDS tables, all generated in FP64 on the GPU and laid
+//     out so that the lanes of a wave read consecutive entries: per-pass
+//     tables for the sub-transform twiddles, and a per-thread table (or the
+//     full W_N, or a two-level W_N = A[e & 63] * B[e >> 6]) for the
+//     inter-stage twiddle;
 //   * the last radix pass writes straight to HBM with lanes running along the
 //     contiguous output index.
 #include "common.hpp"
@@ -95,12 +97,12 @@ __device__ __forceinline__ int pre_exp(int u, int r) {
 // reads finish (barrier) before the writes start.
 // Twiddle tables (LDS): PTW: `ta` holds this thread's PRE twiddles at
 // (k*R + r)*NT + tid (built by the kernel with pre_exp); otherwise the
-// whole W_N (FULLTAB) or the two-level A/B pair.  PPW: `ts` is this stage's
+// whole W_N (FULLTAB) or the two-level A/B pair.  `ts` is this stage's
 // per-pass table, W_{NS R}^(jm r) at r*NS + jm (jm runs along the lanes, so
-// both PTW and PPW reads are free of bank conflicts); otherwise `ts` is W_S.
+// both reads are free of bank conflicts).
 template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool TOGLOBAL, bool FULLTAB, bool OOP, bool PTW, bool PPW, typename Sink>
-__device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, int ls, const Cx<T>* ta,
+          bool TOGLOBAL, bool FULLTAB, bool OOP, bool PTW, typename Sink>
+__device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, const Cx<T>* ta,
                                           const Cx<T>* tb, int tid, Sink&& sink) {
     using C = Cx<T>;
     constexpr int n = 1 << LN_T;
@@ -112,7 +114,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
     constexpr int BPT = EPT / R;
     constexpr int ES = CONTIG ? 1 : VEC;
     static_assert(BPT * NT * R == E, "work split");
-    static_assert(!PPW || NS == 1 || NS == 16, "per-pass tables cover the second pass of a stage");
+    static_assert(NS == 1 || NS == 16, "per-pass tables cover the second pass of a stage");
     C v[BPT][R];
     int obase[BPT];
 #pragma unroll
@@ -146,15 +148,8 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
         }
         if (NS > 1) {
             const int jm = j & (NS - 1);
-            if constexpr (PPW) {
 #pragma unroll
-                for (int r = 1; r < R; r++) v[k][r] = ctw<FWD>(v[k][r], ts[r * NS + jm]);
-            } else {
-                // W_{NS*R}^(jm*r) = W_S^(jm*r*S/(NS*R)), S = 2^ls
-                const int step = (1 << ls) / (NS * R);
-#pragma unroll
-                for (int r = 1; r < R; r++) v[k][r] = ctw<FWD>(v[k][r], ts[jm * r * step]);
-            }
+            for (int r = 1; r < R; r++) v[k][r] = ctw<FWD>(v[k][r], ts[r * NS + jm]);
         }
         dft<R, FWD>(v[k]);
         const int jm = j & (NS - 1);
@@ -251,7 +246,7 @@ __device__ __forceinline__ Mfma16W mfma16_weights(int lane) {
 
 template <typename T, int LN_T, int NS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
           bool TOGLOBAL, bool FULLTAB, bool OOP, typename Sink>
-__device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, int ls, const Cx<T>* ta,
+__device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, const Cx<T>* ta,
                                             const Cx<T>* tb, int tid, const Mfma16W* wtab, Sink&& sink) {
     static_assert(sizeof(T) == 4, "split-precision MFMA pass is FP32 only");
     static_assert(OOP || TOGLOBAL, "the MFMA pass writes each group as soon as it is done");
@@ -264,6 +259,9 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
     constexpr int N = 1 << LN;
     constexpr int ES = CONTIG ? 1 : VEC;
     static_assert(NT % 64 == 0 && (NT * EPT) == (1 << LE), "work split");
+    // stages of at most 128 points: the radix-16 pass is always a stage's
+    // first (NS = 1), so it reads no W table (`ts` is the per-pass table)
+    static_assert(NS == 1, "MFMA radix-16 pass with an inter-pass twiddle");
     const int lane = tid & 63, wave = tid >> 6, col = lane & 15, g = lane >> 4;
     // Register budget (the fused kernel runs at up to 256 VGPRs): the groups
     // are not unrolled, and each group reads its A operands from LDS.
@@ -294,11 +292,6 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
                 const int e = ((j + r * nbf) * kk) & (N - 1);
                 if constexpr (FULLTAB) x[i] = ctw<FWD>(x[i], ta[e]);
                 else x[i] = ctw<FWD>(x[i], cmul(ta[e & 63], tb[e >> 6]));
-            }
-            if (NS > 1) {
-                const int jm = j & (NS - 1);
-                const int step = (1 << ls) / (NS * R);
-                if (r > 0) x[i] = ctw<FWD>(x[i], ts[jm * r * step]);
             }
         }
         half8_t bh, bl;
@@ -352,7 +345,7 @@ struct NoSink {
 // value is the buffer that holds the result.
 template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
           bool FULLTAB, bool MF, bool OOP, int HOFF = 0, typename Hook = NoHook>
-__device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const Cx<T>* ts, int ls, const Cx<T>* ta,
+__device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const Cx<T>* ts, const Cx<T>* ta,
                                                   const Cx<T>* tb, int tid, const Mfma16W* w, Hook&& hook = Hook{}) {
     if constexpr (PASS < NPASS) {
         constexpr int R = fast::Radix<LN_T, PASS>::value;
@@ -360,13 +353,13 @@ __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const 
         Cx<T>* dst = OOP ? alt : cur;
         if constexpr (MF && R == 16)
             mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
-                cur, dst, ts, ls, ta, tb, tid, w, NoSink{});
+                cur, dst, ts, ta, tb, tid, w, NoSink{});
         else
             fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP,
-                      kPerThreadPre<MF, FULLTAB, LN, LE>, !MF>(cur, dst, ts, ls, ta, tb, tid, NoSink{});
+                      kPerThreadPre<MF, FULLTAB, LN, LE>>(cur, dst, ts, ta, tb, tid, NoSink{});
         hook(std::integral_constant<int, HOFF + PASS + 1>{});
         return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP, HOFF>(
-            dst, OOP ? cur : alt, ts, ls, ta, tb, tid, w, hook);
+            dst, OOP ? cur : alt, ts, ta, tb, tid, w, hook);
     } else {
         return cur;
     }
@@ -375,14 +368,14 @@ __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const 
 // last pass of a direction, straight to HBM through `sink`
 template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
           bool FULLTAB, bool MF, typename Sink>
-__device__ __forceinline__ void last_pass(const Cx<T>* cur, const Cx<T>* ts, int ls, const Cx<T>* ta,
+__device__ __forceinline__ void last_pass(const Cx<T>* cur, const Cx<T>* ts, const Cx<T>* ta,
                                           const Cx<T>* tb, int tid, const Mfma16W* w, Sink&& sink) {
     if constexpr (MF && R == 16)
-        mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false>(cur, nullptr, ts, ls, ta,
+        mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false>(cur, nullptr, ts, ta,
                                                                                            tb, tid, w, sink);
     else
         fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE, true, FULLTAB, false,
-                  kPerThreadPre<MF, FULLTAB, LN, LE>, !MF>(cur, nullptr, ts, ls, ta, tb, tid, sink);
+                  kPerThreadPre<MF, FULLTAB, LN, LE>>(cur, nullptr, ts, ta, tb, tid, sink);
 }
 
 // Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
@@ -412,13 +405,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     const int tid = threadIdx.x;
 
     // global table layout: [W_S | A (64) | B (N/64) | W_N (N)]
-    // LDS table layout:    [W_S | per-pass V (VEC) | per-pass M (M) | PRE]
+    // LDS table layout:    [per-pass V (VEC) | per-pass M (M) | PRE]
     // PRE: per-thread table (kPerThreadPre, E = N entries), else W_N (FULLTAB),
     // else A | B.  The per-pass tables hold W_{16 R}^(jm r) at r*16 + jm for
     // the second pass (NS = 16, radix R) of the VEC and M stages.
     constexpr bool PTW = kPerThreadPre<MF, FULLTAB, LN, LE>;
-    constexpr int TV = S, TM = S + VEC, TP = S + VEC + M;
-    for (int t = tid; t < S; t += NT) tabs[t] = tables[t];
+    constexpr int TV = 0, TM = VEC, TP = VEC + M;
     if constexpr (NPV > 1) {
         constexpr int RV = fast::Radix<LVEC, 1>::value;
         for (int t = tid; t < 16 * RV; t += NT) tabs[TV + t] = tables[(t & 15) * (t >> 4) * (S / (16 * RV))];
@@ -442,12 +434,11 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     } else {
         for (int t = tid; t < NTWO - S; t += NT) tabs[TP + t] = tables[S + t];
     }
-    const C* ts = tabs;
     const C* ta = tabs + TP;
     const C* tb = tabs + TP + 64;
-    // per-pass tables of the vector-ALU passes; the MFMA path indexes W_S
-    const C* tsv = MF ? ts : tabs + TV;
-    const C* tsm = MF ? ts : tabs + TM;
+    // per-pass tables of each stage's second pass
+    const C* tsv = tabs + TV;
+    const C* tsm = tabs + TM;
     // MF: the per-lane MFMA weights and two slots of per-wave maxima follow
     // the tables in LDS
     Mfma16W* w = reinterpret_cast<Mfma16W*>(tabs + TP + (FULLTAB ? N : NTWO - S));
@@ -582,12 +573,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
             C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0>(
-                lds, lds2, tsv, LS, ta, tb, tid, w, hook);
+                lds, lds2, tsv, ta, tb, tid, w, hook);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
             if constexpr (STAGED) {
                 cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, tsm, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsm, ta, tb, tid, w, hook);
                 // element k2 of transform (line, k1) is at line*N + k1 + k2*VEC;
                 // sweep each destination row (rank gq, position fr + k2*VP)
 #pragma unroll
@@ -602,12 +593,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 }
             } else {
                 cur = fast_passes_lds<T, LM, 0, NPM - 1, LVEC, LN, LE, NT, false, true, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, tsm, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsm, ta, tb, tid, w, hook);
                 constexpr int PL = NPM - 1;
                 constexpr int RL = fast::Radix<LM, PL>::value;
                 constexpr int NSL = 1 << (4 * PL);
                 last_pass<T, LM, NSL, RL, LVEC, LN, LE, NT, false, true, NPM == 1, FULLTAB, MF>(
-                    cur, tsm, LS, ta, tb, tid, w, [&](int u, int k2, C v) {
+                    cur, tsm, ta, tb, tid, w, [&](int u, int k2, C v) {
                         // transform (line, k1): k1 = u & (VEC-1), line = u >> (LVEC + log2(nbf))
                         constexpr int nbf = (1 << LM) / RL;
                         const int k1 = u & (VEC - 1);
@@ -621,11 +612,11 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
             // the conjugate inter-stage twiddle; the last pass stores to HBM
             C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP, 0>(
-                lds, lds2, tsm, LS, ta, tb, tid, w, hook);
+                lds, lds2, tsm, ta, tb, tid, w, hook);
             const T sc = MF ? scale * inv_s0 : scale;
             if constexpr (STAGED) {
                 cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, tsv, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsv, ta, tb, tid, w, hook);
                 // element jj of transform (line, r = gm*B + j) is at
                 // line*N + r*VEC + jj = line*N + gm*L + (j*VEC + jj): the
                 // group's LDS order is the destination rows' order
@@ -639,12 +630,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 }
             } else {
                 cur = fast_passes_lds<T, LVEC, 0, NPV - 1, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP, NP1>(
-                    cur, cur == lds ? lds2 : lds, tsv, LS, ta, tb, tid, w, hook);
+                    cur, cur == lds ? lds2 : lds, tsv, ta, tb, tid, w, hook);
                 constexpr int PL = NPV - 1;
                 constexpr int RL = fast::Radix<LVEC, PL>::value;
                 constexpr int NSL = 1 << (4 * PL);
                 last_pass<T, LVEC, NSL, RL, LVEC, LN, LE, NT, true, false, NPV == 1, FULLTAB, MF>(
-                    cur, tsv, LS, ta, tb, tid, w, [&](int u, int jj, C v) {
+                    cur, tsv, ta, tb, tid, w, [&](int u, int jj, C v) {
                         // transform (line, r): r = bt & (M-1), line = bt >> LM
                         constexpr int nbf = (1 << LVEC) / RL;
                         const int bt = u >> fast::Log2<nbf>::value;
@@ -676,11 +667,11 @@ int lines_per_iter_log2(int n) {
 
 // LDS bytes of the specialised kernel for element size `es`
 size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false) {
-    const int n = vec * m, s = vec > m ? vec : m;
+    const int n = vec * m;
     const int e = n << lines_per_iter_log2(n);
-    // W_S, the two per-pass tables, then W_N (or the per-thread PRE table of
-    // the same size) or the two-level pair
-    const int tab = s + vec + m + (fulltab ? n : 64 + (n / 64 > 0 ? n / 64 : 1));
+    // the two per-pass tables, then W_N (or the per-thread PRE table of the
+    // same size) or the two-level pair
+    const int tab = vec + m + (fulltab ? n : 64 + (n / 64 > 0 ? n / 64 : 1));
     const int shift = n >= 4096 ? (es == 16 ? 6 : 4) : 5;
     const int padded = e + (e >> shift) + 1;
     return es * (padded * (mf ? 2 : 1) + tab) + (mf ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0);
