@@ -541,3 +541,31 @@ def test_plan_path_query():
     assert p2.two_stage and not p2.fast
     p3 = plan(2, 64, 2, 2, "fp32")  # M = 4: generic line pass
     assert not p3.fast and not p3.two_stage
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_concurrent_streams_with_own_scratch(precision):
+    """Calls on one plan from two streams at once, each with its own scratch
+    buffer, give the same bits as the same calls made one after the other."""
+    from stencilfft_amd.fields import uniform_fields
+    note(f"two streams {precision}")
+    pl = plan(2, 1024, 2, 16, precision)
+    xs = [uniform_fields(pl, 70 + i) for i in range(2)]
+    want = []
+    for x in xs:
+        y = pl.forward(x)
+        want.append((y, pl.inverse(y, 1.0 / 1024 ** 2)))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    scratch = [torch.empty(pl.scratch_bytes // xs[0].element_size(), dtype=xs[0].dtype, device="cuda")
+               for _ in range(2)]
+    outs = [(torch.empty_like(x), torch.empty_like(x)) for x in xs]
+    for s in streams:
+        s.wait_stream(torch.cuda.current_stream())
+    for _ in range(5):  # repeated so that the two streams' kernels overlap
+        for i, s in enumerate(streams):
+            pl.forward(xs[i], out=outs[i][0], scratch=scratch[i], stream=s)
+            pl.inverse(outs[i][0], 1.0 / 1024 ** 2, out=outs[i][1], scratch=scratch[i], stream=s)
+    torch.cuda.synchronize()
+    for (y, z), (wy, wz) in zip(outs, want):
+        assert torch.equal(y, wy) and torch.equal(z, wz)
