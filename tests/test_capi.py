@@ -110,6 +110,28 @@ def test_plan_create_validates_before_touching_the_device():
         assert L._lib().zm_plan_create(ctypes.byref(h), ctypes.byref(good), 0, 1, 0) == L.ZM_ERR_DEVICE
 
 
+def _plan_entry_points():
+    from stencilfft_amd import _lib as L
+    return sorted(name for name, (_, args) in L.SIGNATURES.items()
+                  if args and args[0] is ctypes.c_void_p and name != "zm_plan_create")
+
+
+@pytest.mark.parametrize("name", _plan_entry_points())
+def test_null_plan_is_rejected_before_any_device_call(name):
+    # Every entry point that takes a plan checks it first: with a NULL plan
+    # (and NULL buffers) it returns ZM_ERR_INVALID_ARG, or 0 bytes for the
+    # size queries, without touching the device or the other arguments.
+    from stencilfft_amd import _lib as L
+    res, args = L.SIGNATURES[name]
+    vals = [0 if a in (ctypes.c_int, ctypes.c_uint) else 1.0 if a is ctypes.c_double else None
+            for a in args]
+    got = getattr(L._lib(), name)(*vals)
+    if name in ("zm_scratch_bytes", "zm_field_bytes"):
+        assert got == 0
+    else:
+        assert got == L.ZM_ERR_INVALID_ARG
+
+
 def test_product_does_not_import_the_oracle():
     # The product package must never route through the checker.
     pkg = os.path.join(ROOT, "stencilfft_amd")

@@ -6,6 +6,7 @@ bit-exact; FP64 stencils are bit-exact (same evaluation order, no FMA
 contraction); FP64 transforms within 1e-13 relative max error; FP32 transforms
 within the 1e-6 relative-L2 budget of the FP32 path (max error 1e-5).
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -464,6 +465,22 @@ def test_error_paths():
     with pytest.raises(ZmError) as e:
         plan(2, 1024, 2, 16, "fp32_mfma", two_stage=True)
     assert e.value.status == 3
+    # The C ABI with a live plan: NULL buffers, a scratch buffer that aliases
+    # an operand, an unknown element kind and a zero or NaN step are rejected
+    # before any launch.
+    from stencilfft_amd import _lib as L
+    lib, h = L._lib(), pl.handle
+    y = torch.zeros_like(x)
+    xp, yp = ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr())
+    assert lib.zm_forward(h, None, yp, None, None) == L.ZM_ERR_INVALID_ARG
+    assert lib.zm_inverse(h, xp, None, 1.0, None, None) == L.ZM_ERR_INVALID_ARG
+    assert lib.zm_forward(h, xp, yp, xp, None) == L.ZM_ERR_ALIAS
+    assert lib.zm_laplace(h, xp, None, 1.0, 0, None) == L.ZM_ERR_INVALID_ARG
+    assert lib.zm_laplace(h, xp, yp, 1.0, 7, None) == L.ZM_ERR_INVALID_ARG
+    assert lib.zm_dx4(h, xp, yp, 0.0, 0, None) == L.ZM_ERR_INVALID_ARG
+    assert lib.zm_dy4(h, xp, yp, float("nan"), 0, None) == L.ZM_ERR_INVALID_ARG
+    assert lib.zm_natural_to_packed(h, None, yp, None) == L.ZM_ERR_INVALID_ARG
+    torch.cuda.synchronize()
 
 
 @pytest.mark.parametrize("n,p,b,kmax", [(2048, 4, 8, 16), (4096, 4, 16, 512)])
