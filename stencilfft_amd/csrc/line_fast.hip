@@ -402,7 +402,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     C* lds = reinterpret_cast<C*>(smem_raw);
     C* lds2 = OOP ? lds + fast_padded<T, LN>(E) : lds;
     C* tabs = lds + fast_padded<T, LN>(E) * (OOP ? 2 : 1);
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
 
     // global table layout: [W_S | A (64) | B (N/64) | W_N (N)]
     // LDS table layout:    [per-pass V (VEC) | per-pass M (M) | PRE]
@@ -536,6 +536,13 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     }
     __syncthreads();  // tables, maxima
     for (; grp < groups; grp += gridDim.x, ++it) {
+        // FP64, N = 8192: the compiler hoists every per-thread global and LDS
+        // address out of this loop and keeps them live, which spills 6 to 87
+        // VGPRs at the 256-VGPR limit.  An opaque tid per iteration makes
+        // them recomputed per group instead (154-163 VGPRs, no spills; the
+        // pass 0.562 -> 0.467 ms).  Shorter lines do not spill, and there the
+        // recomputation only costs (3D N = 512: 0.725 -> 0.835 ms).
+        if constexpr (sizeof(T) == 8 && LN >= 13) asm volatile("" : "+v"(tid));
         if constexpr (MF) {
             float m = 0.0f;
 #pragma unroll

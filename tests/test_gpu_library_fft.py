@@ -20,6 +20,8 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a
 CASES = [
     (2, 1024, 2, 16, "fp64"), (2, 1024, 2, 16, "fp32"), (2, 8192, 4, 32, "fp32"), (2, 4096, 4, 16, "fp64"),
     (3, 128, 2, 8, "fp64"), (3, 512, 2, 16, "fp32"), (2, 2048, 2, 16, "fp32_mfma"), (2, 8192, 4, 32, "fp32_mfma"),
+    # FP64 line passes of N = 8192 in both (VEC, M) orders: (64, 128) and (128, 64)
+    (2, 8192, 4, 32, "fp64"), (2, 8192, 4, 16, "fp64"),
 ]
 
 
