@@ -272,7 +272,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs the whole P x P virtual rank grid on one GPU; "
+        print(json.dumps({"error": "stencilfft-amd runs the whole P x P virtual rank grid on one GPU; "
                                    "--gpus must be 1 (multi-GPU sharding is a design note, not built)"}))
         sys.exit(2)
 

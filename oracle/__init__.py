@@ -1,6 +1,6 @@
 """CPU FP64 oracle for StencilFFT's zmodel hot path -- TEST INFRASTRUCTURE ONLY.
 
-This package is the checker.  Only tests/, ``__graft_entry__.smoke()`` and the
+This package is the oracle.  Only tests/, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` import it; the product package
 (``stencilfft_amd``) never does.  It wraps ``oracle/liboracle.so`` (built from
 ``oracle/oracle.c`` by ``oracle/Makefile``), a plain-C restatement of the

@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU FP64 restatement of StencilFFT's zmodel hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Loaded by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
  * product.  Each function cites the reference file:line it restates.  The
  * reference's MPI ranks are the slabs of one virtual-grid array (see oracle.h).

@@ -1,7 +1,7 @@
 /*
  * oracle.h -- CPU FP64 restatement of StencilFFT's zmodel hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by tests/,
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: it is loaded by tests/,
  * by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by nothing
  * else.  The product (stencilfft_amd/) never links, loads or calls it.
  *

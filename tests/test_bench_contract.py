@@ -1,5 +1,5 @@
 """CPU checks of bench.py's command-line contract: one GPU only, one JSON line
-on every exit path, and the entry points the round-end tooling calls."""
+on every exit path, and the build / smoke entry points."""
 import json
 import os
 import subprocess

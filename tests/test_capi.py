@@ -133,7 +133,7 @@ def test_null_plan_is_rejected_before_any_device_call(name):
 
 
 def test_product_does_not_import_the_oracle():
-    # The product package must never route through the checker.
+    # The product package must never route through the oracle.
     pkg = os.path.join(ROOT, "stencilfft_amd")
     for dirpath, _, files in os.walk(pkg):
         for f in files:
