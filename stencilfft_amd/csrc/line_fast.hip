@@ -715,11 +715,18 @@ static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, co
 
 // Dispatch over the instantiated (VEC, M) pairs; `op` is called with the
 // kernel's template arguments.
+// The FP32 vector-ALU pass always has room for the full W_N table (at most
+// 136.7 KB of LDS at N = 8192), so its two-level variants are not built.
+template <typename T, bool MF>
+constexpr bool kBuildTwoLevel = !(sizeof(T) == 4 && !MF);
+
 template <typename T, bool MF, typename Op>
 static auto with_fast(int vec, int m, bool fwd, Op&& op) {
-#define ZM_LF1(V, MM, F, FT)                                                          \
-    if (vec == (1 << V) && m == (1 << MM) && fwd == F && fast_line_fulltab<T>(vec, m, MF) == FT) \
-        return op.template run<V, MM, ((V + MM) >= 12 ? 0 : 12 - (V + MM)), F, FT, MF>();
+#define ZM_LF1(V, MM, F, FT)                                                                           \
+    if constexpr (FT || kBuildTwoLevel<T, MF>) {                                                       \
+        if (vec == (1 << V) && m == (1 << MM) && fwd == F && fast_line_fulltab<T>(vec, m, MF) == FT) \
+            return op.template run<V, MM, ((V + MM) >= 12 ? 0 : 12 - (V + MM)), F, FT, MF>();         \
+    }
 #define ZM_LF(V, MM) ZM_LF1(V, MM, true, true) ZM_LF1(V, MM, true, false) ZM_LF1(V, MM, false, true) ZM_LF1(V, MM, false, false)
     ZM_LF(4, 4) ZM_LF(4, 5) ZM_LF(4, 6) ZM_LF(4, 7)
     ZM_LF(5, 4) ZM_LF(5, 5) ZM_LF(5, 6) ZM_LF(5, 7)
