@@ -536,13 +536,18 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     }
     __syncthreads();  // tables, maxima
     for (; grp < groups; grp += gridDim.x, ++it) {
-        // FP64, N = 8192: the compiler hoists every per-thread global and LDS
-        // address out of this loop and keeps them live, which spills 6 to 87
-        // VGPRs at the 256-VGPR limit.  An opaque tid per iteration makes
-        // them recomputed per group instead (154-163 VGPRs, no spills; the
-        // pass 0.562 -> 0.467 ms).  Shorter lines do not spill, and there the
-        // recomputation only costs (3D N = 512: 0.725 -> 0.835 ms).
-        if constexpr (sizeof(T) == 8 && LN >= 13) asm volatile("" : "+v"(tid));
+        // The compiler hoists every per-thread global and LDS address out of
+        // this loop and keeps them live.  An opaque tid per iteration makes
+        // them recomputed per group instead, where that was measured faster:
+        //  * FP64, N = 8192: the hoisted addresses spill 6 to 87 VGPRs at the
+        //    256-VGPR limit; recomputed, 154-163 VGPRs and no spills (the
+        //    pass 0.562 -> 0.467 ms);
+        //  * FP32 vector ALU, several lines per group: 228 -> 102-120 VGPRs,
+        //    four 256-thread workgroups per CU instead of two (3D N = 512:
+        //    0.413 -> 0.401 ms).
+        // Elsewhere it only costs (FP64 3D N = 512: 0.725 -> 0.835 ms).
+        constexpr bool kRecompute = (sizeof(T) == 8 && LN >= 13) || (sizeof(T) == 4 && !MF && LLPI > 0);
+        if constexpr (kRecompute) asm volatile("" : "+v"(tid));
         if constexpr (MF) {
             float m = 0.0f;
 #pragma unroll
