@@ -560,6 +560,20 @@ def test_plan_path_query():
     assert not p3.fast and not p3.two_stage
 
 
+SPECIALISED_PAIRS = [(v, m) for v in (16, 32, 64, 128) for m in (16, 32, 64, 128) if v * m <= 8192]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32_mfma"])
+def test_every_specialised_pair_takes_the_fused_pass(precision):
+    # N = VEC * M with P = 2 and B = M / 2 gives exactly that (VEC, M); every
+    # pair the specialised pass is built for must be selected, in each
+    # precision (a dispatch gap would fall back to the generic pass silently)
+    note(f"fused pass for every (VEC, M) pair, {precision}")
+    for vec, m in SPECIALISED_PAIRS:
+        p = plan(2, vec * m, 2, m // 2, precision)
+        assert p.fast, (vec, m)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_concurrent_streams_with_own_scratch(precision):
     """Calls on one plan from two streams at once, each with its own scratch
