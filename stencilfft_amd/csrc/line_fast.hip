@@ -562,13 +562,15 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         const int nxt = grp + gridDim.x;
         // The next group's loads are spread over the pass boundaries (HN
         // points: after staging and after every LDS pass before the last); a
-        // burst of EPT loads per thread stalls instruction issue.  FP32
-        // (vector ALU and MFMA) always; FP64 for one-line groups (N >= 4096:
-        // 0.466 -> 0.401 ms at N = 8192), not for the multi-line groups of
-        // shorter lines, which run one wave per SIMD and were slower spread
-        // (3D N = 512: 0.725 -> 0.871 ms).  MF's maxima of the next group
-        // are taken after the last pass, when all of its loads were issued.
-        constexpr bool SPREAD = sizeof(T) == 4 || LLPI == 0;
+        // burst of EPT loads per thread stalls instruction issue.  Measured:
+        //  * FP32 vector ALU: always (N = 8192: 0.234 -> 0.213 ms);
+        //  * one-line groups (N >= 4096) of FP64 (N = 8192: 0.466 -> 0.401
+        //    ms) and MFMA (0.289 -> 0.260 ms);
+        //  * not the multi-line groups of FP64 or MFMA, which were slower
+        //    spread (3D N = 512: FP64 0.725 -> 0.871 ms, MFMA 0.456 -> 0.470).
+        // MF's maxima of the next group are taken after the last pass, when
+        // all of its loads were issued.
+        constexpr bool SPREAD = (sizeof(T) == 4 && !MF) || LLPI == 0;
         constexpr int NP1 = FWD ? NPV : NPM;
         constexpr int NP2 = STAGED ? (FWD ? NPM : NPV) : ((FWD ? NPM : NPV) - 1);
         constexpr int HN = 1 + NP1 + NP2;
