@@ -21,7 +21,9 @@
  *     asynchronous, nothing synchronises the device;
  *   - in and out must not alias unless an entry point says otherwise;
  *   - a plan is immutable after creation; concurrent calls on one plan must
- *     use distinct scratch buffers.
+ *     use distinct scratch buffers.  Calls that pass scratch = NULL share the
+ *     buffer the plan allocated at creation, so they must be ordered on one
+ *     stream (or otherwise serialised by the caller).
  */
 #ifndef STENCILFFT_H
 #define STENCILFFT_H
@@ -54,6 +56,8 @@ typedef enum {
 /* plan flags */
 #define ZM_FLAG_TWO_STAGE 1u /* per direction run FFT1a+pack+exchange, then unpack+twiddle+FFT1b,
                                 materialising the reference's post-exchange buffer */
+#define ZM_FLAG_CALLER_SCRATCH 2u /* the plan allocates no scratch of its own: every transform call
+                                     passes one (scratch = NULL returns ZM_ERR_INVALID_ARG) */
 
 /* The shape a plan is built for: zmodel/utils.h:28-71 derives everything else. */
 typedef struct {
@@ -86,7 +90,9 @@ typedef struct zm_plan zm_plan;
 zm_status zm_shape_check(const zm_shape* shape, zm_dims* out);
 
 /* Create a plan on the current HIP device: validates the shape, builds FP64
- * twiddle tables on the GPU and picks the kernel path.  Replaces
+ * twiddle tables on the GPU, picks the kernel path and, unless `flags` has
+ * ZM_FLAG_CALLER_SCRATCH, allocates the plan's own scratch
+ * (zm_scratch_bytes).  Unknown flag bits are ZM_ERR_INVALID_ARG.  Replaces
  * init_plans / init_plans_3d (zmodel/transforms/fft_host_plans.h:14-61,
  * 75-121), init_fftdx_fft1 (fft_device_plans.h:37-46) and the twiddle half of
  * init_buffers / init_buffers_3d (zmodel/transforms/fft.cu:75-108,
@@ -119,7 +125,9 @@ size_t zm_field_bytes(const zm_plan* plan);
  * Replaces forward_fft<true> (zmodel/transforms/fft.h:36-38, fft.cu:199-339)
  * and forward_fft_3d<true> (fft.h:54-56, fft3d.cu:234-412).  Unlike the
  * reference, `in` is not clobbered.  `scratch` holds zm_scratch_bytes(plan)
- * bytes, or is NULL to use a buffer the plan owns. */
+ * bytes and overlaps neither `in` nor `out` (only pointer equality is
+ * checked here), or is NULL to use the buffer the plan allocated at creation
+ * (ZM_ERR_INVALID_ARG for a ZM_FLAG_CALLER_SCRATCH plan). */
 zm_status zm_forward(zm_plan* plan, const void* in, void* out, void* scratch, void* stream);
 
 /* Inverse (unnormalised inverse DFTs, `scale` applied once), reciprocal ->

@@ -125,7 +125,24 @@ def build_hdf5() -> str | None:
     if res.returncode != 0:
         raise RuntimeError(f"HDF5 I/O build failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, HDF5_LIB)
+    build_hdf5_example()
     return HDF5_LIB
+
+
+HDF5_EXAMPLE_SRC = os.path.join(HERE, "examples", "hdf5_example.cpp")
+HDF5_EXAMPLE_BIN = os.path.join(HERE, "examples", "zmodel_hdf5_example")
+
+
+def build_hdf5_example() -> str:
+    """A field through the HDF5 file format and the GPU, over the zmodel.hpp
+    facade (linked against both libraries, rpath $ORIGIN/..)."""
+    include = os.path.join(os.path.dirname(HERE), "include")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", HDF5_EXAMPLE_SRC, f"-I{include}", f"-L{HERE}",
+           "-lstencilfft_hdf5", "-lstencilfft", "-Wl,-rpath,$ORIGIN/..", "-o", HDF5_EXAMPLE_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"HDF5 example build failed:\n{res.stdout}\n{res.stderr}")
+    return HDF5_EXAMPLE_BIN
 
 
 if __name__ == "__main__":

@@ -554,6 +554,10 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             for (int v = 0; v < NW; v++) m = fmaxf(m, wmax[(it & 1) * 16 + v]);
             int ex;
             (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+            // a group of FP32 denormals or tiny normals would need a scale
+            // beyond 2^127: clamp, so s0 stays finite (its largest value
+            // then lands below 2^7, which costs no range)
+            ex = ex < MFMA_SCALE_LOG2 - 127 ? MFMA_SCALE_LOG2 - 127 : ex;
             s0 = ldexpf(1.0f, MFMA_SCALE_LOG2 - ex);
             inv_s0 = ldexpf(1.0f, ex - MFMA_SCALE_LOG2);
         }

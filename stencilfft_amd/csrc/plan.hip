@@ -159,6 +159,7 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
     *plan = nullptr;
     if (precision != ZM_FP64 && precision != ZM_FP32 && precision != ZM_FP32_MFMA) return ZM_ERR_INVALID_ARG;
     if (batch < 1 || batch > 65535) return ZM_ERR_INVALID_ARG;
+    if (flags & ~(ZM_FLAG_TWO_STAGE | ZM_FLAG_CALLER_SCRATCH)) return ZM_ERR_INVALID_ARG;
     zm_dims d;
     ZM_TRY(zm_shape_check(shape, &d));
     const bool mfma = precision == ZM_FP32_MFMA;
@@ -224,12 +225,21 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
     g.lvec = ilog2(d.vec);
     p->scratch_bytes = (size_t)batch * (size_t)d.field * es;
 
+    // The plan's own scratch is allocated here, once, so that the plan is
+    // immutable after creation; ZM_FLAG_CALLER_SCRATCH plans own none and
+    // every transform call must bring one.
+    if (!(flags & ZM_FLAG_CALLER_SCRATCH) && hipMalloc(&p->own_scratch, p->scratch_bytes) != hipSuccess) {
+        delete p;
+        return ZM_ERR_NOMEM;
+    }
     if (hipMalloc(&p->tw64, sizeof(double2) * n) != hipSuccess) {
+        if (p->own_scratch) (void)hipFree(p->own_scratch);
         delete p;
         return ZM_ERR_NOMEM;
     }
     if (hipMalloc(&p->tw32, sizeof(float2) * n) != hipSuccess) {
         (void)hipFree(p->tw64);
+        if (p->own_scratch) (void)hipFree(p->own_scratch);
         delete p;
         return ZM_ERR_NOMEM;
     }
@@ -261,6 +271,7 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
         (void)hipFree(p->tw32);
         if (p->ltab64) (void)hipFree(p->ltab64);
         if (p->ltab32) (void)hipFree(p->ltab32);
+        if (p->own_scratch) (void)hipFree(p->own_scratch);
         delete p;
         return e != hipSuccess ? ZM_ERR_DEVICE : ZM_ERR_UNSUPPORTED;
     }
@@ -462,19 +473,11 @@ zm_status check_io(const zm_plan* p, const void* in, const void* out) {
     return ZM_OK;
 }
 
-zm_status scratch_for(zm_plan* p, void* scratch, void** s) {
-    if (scratch) {
-        *s = scratch;
-        return ZM_OK;
-    }
-    if (!p->own_scratch) {
-        if (hipMalloc(&p->own_scratch, p->scratch_bytes) != hipSuccess) {
-            p->own_scratch = nullptr;
-            return ZM_ERR_NOMEM;
-        }
-    }
-    *s = p->own_scratch;
-    return ZM_OK;
+// The caller's scratch, else the one the plan allocated at creation (none
+// for ZM_FLAG_CALLER_SCRATCH plans: then NULL is an invalid argument).
+zm_status scratch_for(const zm_plan* p, void* scratch, void** s) {
+    *s = scratch ? scratch : p->own_scratch;
+    return *s ? ZM_OK : ZM_ERR_INVALID_ARG;
 }
 
 zm_status run_transform(zm_plan* p, const void* in, void* out, void* scratch, void* stream, bool fwd, bool compute,

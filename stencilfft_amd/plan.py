@@ -117,12 +117,13 @@ class Plan:
     """
 
     def __init__(self, dims: int, n: int, p: int, b: int, precision: str = "fp64", batch: int = 1,
-                 two_stage: bool = False):
+                 two_stage: bool = False, caller_scratch: bool = False):
         self.shape = Shape(dims, n, p, b)
         self.dims = self.shape.check()
         self.precision = precision
         self.batch = batch
-        flags = L.ZM_FLAG_TWO_STAGE if two_stage else 0
+        self.caller_scratch = caller_scratch
+        flags = (L.ZM_FLAG_TWO_STAGE if two_stage else 0) | (L.ZM_FLAG_CALLER_SCRATCH if caller_scratch else 0)
         h = ctypes.c_void_p()
         L.check(L._lib().zm_plan_create(ctypes.byref(h), ctypes.byref(self.shape._c()), PRECISIONS[precision],
                                        batch, flags), f"zm_plan_create({self.shape}, {precision})")
@@ -189,44 +190,77 @@ class Plan:
             self._check_tensor(out, real)
         return out
 
+    @staticmethod
+    def _span(t):
+        start = t.data_ptr()
+        return start, start + t.numel() * t.element_size()
+
+    def _scratch(self, scratch, *tensors):
+        """Pointer of a caller's scratch tensor after checking it: a contiguous
+        CUDA tensor on the input's device, at least scratch_bytes long, that
+        overlaps none of `tensors`.  None selects the plan's own scratch (an
+        error for a caller-scratch plan).  Raises before anything is launched."""
+        if scratch is None:
+            if self.caller_scratch:
+                raise ValueError("this plan owns no scratch (caller_scratch=True): pass scratch=")
+            return None
+        nbytes = scratch.numel() * scratch.element_size()
+        if not scratch.is_cuda or not scratch.is_contiguous() or nbytes < self.scratch_bytes:
+            raise ValueError(f"scratch must be a contiguous CUDA tensor of at least {self.scratch_bytes} bytes, got "
+                             f"{tuple(scratch.shape)} {scratch.dtype} ({nbytes} bytes) on {scratch.device}")
+        if tensors and scratch.device != tensors[0].device:
+            raise ValueError(f"scratch is on {scratch.device}, the fields on {tensors[0].device}")
+        s0, s1 = self._span(scratch)
+        for t in tensors:
+            t0, t1 = self._span(t)
+            if s0 < t1 and t0 < s1:
+                raise ValueError("scratch overlaps an input or output field")
+        return scratch.data_ptr()
+
     # ---- transforms ----
     def forward(self, x, out=None, scratch=None, stream=None):
         """forward_fft (zmodel/transforms/fft.cu:199-339 / fft3d.cu:234-412)."""
         out = self._out(x, out)
-        L.check(L._lib().zm_forward(self._h, _ptr(x), _ptr(out), _ptr(scratch), _stream(stream)), "zm_forward")
+        sp = self._scratch(scratch, x, out)
+        L.check(L._lib().zm_forward(self._h, _ptr(x), _ptr(out), sp, _stream(stream)), "zm_forward")
         return out
 
     def inverse(self, x, scale: float = 1.0, out=None, scratch=None, stream=None):
         """inverse_fft (zmodel/transforms/fft.cu:409-514 / fft3d.cu:508-633)."""
         out = self._out(x, out)
-        L.check(L._lib().zm_inverse(self._h, _ptr(x), _ptr(out), float(scale), _ptr(scratch), _stream(stream)),
+        sp = self._scratch(scratch, x, out)
+        L.check(L._lib().zm_inverse(self._h, _ptr(x), _ptr(out), float(scale), sp, _stream(stream)),
                 "zm_inverse")
         return out
 
     def permute_forward(self, x, out=None, scratch=None, stream=None):
         """forward_fft<false>: the data movement alone."""
         out = self._out(x, out)
-        L.check(L._lib().zm_permute_forward(self._h, _ptr(x), _ptr(out), _ptr(scratch), _stream(stream)),
+        sp = self._scratch(scratch, x, out)
+        L.check(L._lib().zm_permute_forward(self._h, _ptr(x), _ptr(out), sp, _stream(stream)),
                 "zm_permute_forward")
         return out
 
     def permute_inverse(self, x, out=None, scratch=None, stream=None):
         """inverse_fft<false>: the data movement alone."""
         out = self._out(x, out)
-        L.check(L._lib().zm_permute_inverse(self._h, _ptr(x), _ptr(out), _ptr(scratch), _stream(stream)),
+        sp = self._scratch(scratch, x, out)
+        L.check(L._lib().zm_permute_inverse(self._h, _ptr(x), _ptr(out), sp, _stream(stream)),
                 "zm_permute_inverse")
         return out
 
     def forward_adjoint(self, x, out=None, scratch=None, stream=None):
         out = self._out(x, out)
-        L.check(L._lib().zm_forward_adjoint(self._h, _ptr(x), _ptr(out), _ptr(scratch), _stream(stream)),
+        sp = self._scratch(scratch, x, out)
+        L.check(L._lib().zm_forward_adjoint(self._h, _ptr(x), _ptr(out), sp, _stream(stream)),
                 "zm_forward_adjoint")
         return out
 
     def inverse_adjoint(self, x, scale: float = 1.0, out=None, scratch=None, stream=None):
         out = self._out(x, out)
-        L.check(L._lib().zm_inverse_adjoint(self._h, _ptr(x), _ptr(out), float(scale), _ptr(scratch),
-                                           _stream(stream)), "zm_inverse_adjoint")
+        sp = self._scratch(scratch, x, out)
+        L.check(L._lib().zm_inverse_adjoint(self._h, _ptr(x), _ptr(out), float(scale), sp, _stream(stream)),
+                "zm_inverse_adjoint")
         return out
 
     # ---- stencils ----

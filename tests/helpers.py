@@ -8,6 +8,11 @@ import numpy as np
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
+# Relative max-error bars (largest |error| over largest |reference|) of the two
+# FP32 paths against FP64; both also keep the 1e-6 relative-L2 bar.
+FP32_REL_MAX = 1e-5
+MFMA_REL_MAX = 1e-5
+
 
 def load_images():
     """Parse tests/golden/images_n16_p2_b2.txt -> {step: {rank_name: (8, 8) int array}}."""

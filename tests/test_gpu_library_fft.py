@@ -8,12 +8,14 @@ scattered to natural order with the reciprocal index table
 (zmodel/utils.h:254-320); the inverse takes the packed spectrum back.
 
 Bars: FP64 <= 1e-13 relative to the largest output; FP32 <= 1e-6 relative L2
-against the FP64 library result (the library's own FP32 error is printed for
-comparison).
+against the FP64 library result, and relative max error within the bar of the
+FP32 path (FP32_REL_MAX, MFMA_REL_MAX in helpers.py); the library's own FP32
+error is printed for comparison.
 """
 import pytest
 
 from conftest import has_gpu
+from helpers import FP32_REL_MAX, MFMA_REL_MAX
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a GPU")]
 
@@ -22,6 +24,8 @@ CASES = [
     (3, 128, 2, 8, "fp64"), (3, 512, 2, 16, "fp32"), (2, 2048, 2, 16, "fp32_mfma"), (2, 8192, 4, 32, "fp32_mfma"),
     # FP64 line passes of N = 8192 in both (VEC, M) orders: (64, 128) and (128, 64)
     (2, 8192, 4, 32, "fp64"), (2, 8192, 4, 16, "fp64"),
+    # the 3D configuration, 512^3 P=2 B=16, in FP64
+    (3, 512, 2, 16, "fp64"),
 ]
 
 
@@ -60,10 +64,11 @@ def test_forward_inverse_match_library_fft(dims, n, p, b, precision):
         lib = torch.fft.fftn(nat.reshape((n,) * dims), dim=tuple(range(dims))).reshape(-1)
         lib32 = _rel(lib, ref, torch)
     print(f"forward vs FP64 library: rel L2 {l2:.3e}, rel max {mx:.3e}; FP32 library {lib32}", flush=True)
+    bar = MFMA_REL_MAX if precision == "fp32_mfma" else FP32_REL_MAX
     if precision == "fp64":
         assert mx <= 1e-13
     else:
-        assert l2 <= 1e-6
+        assert l2 <= 1e-6 and mx <= bar
     # inverse of the packed spectrum of the library transform, scaled 1/N^d
     spec = ref.to(plan.complex_dtype)[tab].contiguous()
     back = plan.packed_to_natural(plan.inverse(spec, 1.0 / float(n) ** dims))
@@ -72,4 +77,4 @@ def test_forward_inverse_match_library_fft(dims, n, p, b, precision):
     if precision == "fp64":
         assert mxi <= 1e-13
     else:
-        assert l2i <= 1e-6
+        assert l2i <= 1e-6 and mxi <= bar
