@@ -187,49 +187,166 @@ __global__ void __launch_bounds__(256) laplace_kernel(const E* __restrict__ in, 
 }
 
 // [1, -8, 0, 8, -1] / (12 delta) in the reference's order (dx.cu:29-79):
-// s = -8 f(-1) + f(-2) + 8 f(+1) - f(+2), then s / (12 delta).  AXIS 0 = x
-// (packed columns), 1 = y (block-cyclic rows), 2 = z (block-cyclic depth, 3D
-// only; the reference has no z derivative).  The other coordinates are kept,
-// so the 2D kernels are the 3D ones on a one-plane slab.
-template <typename E, typename S, int AXIS>
-__global__ void __launch_bounds__(256) deriv4_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, S denom,
-                                                     int negate) {
-#pragma clang fp contract(off)
-    using O = EOps<E>;
-    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= g.field) return;
-    const E* src = in + (int64_t)blockIdx.y * g.field;
-    const Pt p = decode(g, (int)i);
-    E f[4];  // f(-1), f(-2), f(+1), f(+2)
-    const int steps[4] = {-1, -2, 1, 2};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        int a, b2;
-        if (AXIS == 0) {
-            x_walk(g, p.cid, p.lc >> g.lvec, p.lc & (g.vec - 1), steps[k], &a, &b2);
-            f[k] = src[encode(g, p.did, p.rid, a, p.ld, p.lr, b2)];
-        } else if (AXIS == 1) {
-            bc_walk(g, p.rid, p.lr, steps[k], &a, &b2);
-            f[k] = src[encode(g, p.did, a, p.cid, p.ld, b2, p.lc)];
-        } else {
-            bc_walk(g, p.did, p.ld, steps[k], &a, &b2);
-            f[k] = src[encode(g, a, p.rid, p.cid, b2, p.lr, p.lc)];
-        }
-    }
+// s = -8 f(-1) + f(-2) + 8 f(+1) - f(+2), then s / (12 delta).  FP32 uses
+// 8 (f(+1) - f(-1)) - (f(+2) - f(-2)): differences of nearby values first.
+template <typename O, typename E, typename S>
+__device__ __forceinline__ E deriv4_combine(E fm1, E fm2, E fp1, E fp2, S denom, bool negate) {
     E s;
-    if (sizeof(S) == 8) {
+    if constexpr (sizeof(S) == 8) {
         s = O::zero();
-        s = O::add(s, O::mul(f[0], (S)-8));
-        s = O::add(s, f[1]);
-        s = O::add(s, O::mul(f[2], (S)8));
-        s = O::sub(s, f[3]);
+        s = O::add(s, O::mul(fm1, (S)-8));
+        s = O::add(s, fm2);
+        s = O::add(s, O::mul(fp1, (S)8));
+        s = O::sub(s, fp2);
     } else {
-        // FP32: 8 (f(+1) - f(-1)) - (f(+2) - f(-2)), differences of nearby values first
-        s = O::sub(O::mul(O::sub(f[2], f[0]), (S)8), O::sub(f[3], f[1]));
+        s = O::sub(O::mul(O::sub(fp1, fm1), (S)8), O::sub(fp2, fm2));
     }
     s = O::div(s, denom);
     if (negate) s = O::mul(s, (S)-1);
-    out[(int64_t)blockIdx.y * g.field + i] = s;
+    return s;
+}
+
+// 4th-order derivative as a march along the derivative's global axis.  AXIS
+// 0 = x (packed columns), 1 = y (block-cyclic rows), 2 = z (block-cyclic
+// depth, 3D only; the reference has no z derivative).  Every thread owns one
+// line of that axis and walks `T` consecutive global positions of it:
+//   x: one packed lane jj of one global row; position q in [0, B P) is vector
+//      q mod B of column rank q / B, so the march crosses the P ranks of the
+//      row, and the halos before q = 0 and after q = B P - 1 are the
+//      neighbouring lanes' ends (the reference's lane shift at the global x
+//      edge, dx.cu:32,57, is this wrap of the global column jj B P + q);
+//   y: one packed column lc of column rank cid (and plane gd in 3D), over
+//      global rows, each mapped to its row rank and local row (dy.cu:34-97);
+//   z: one (global row, column rank, lc) pencil over global planes.
+// Consecutive threads own consecutive packed columns (lanes for x), so every
+// load and store of a wave is one contiguous run.  The line sits in a
+// 16-slot register ring that runs 12 positions ahead of the output: each
+// value is loaded once per strip (plus 4 halo loads at the strip ends), 8
+// loads are in flight while one output is computed, and no value is ever
+// exchanged between threads.  Strips of one line are consecutive blocks in
+// XCD-aware order, so a strip's halo loads hit the L2 that its neighbour
+// strip's loads filled.
+template <typename E, typename S, int AXIS>
+__global__ void __launch_bounds__(256) deriv4_march_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g,
+                                                           int T, int lstrips, int lines, S denom, int negate) {
+#pragma clang fp contract(off)
+    using O = EOps<E>;
+    constexpr int RING = 16, AHEAD = 12;  // AHEAD < RING: the slot loaded holds a dead value
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int s = (int)(blk & ((1u << lstrips) - 1));
+    const int o = (int)(blk >> lstrips) * 256 + (int)threadIdx.x;
+    if (o >= lines) return;
+    const int64_t fb = (int64_t)blockIdx.y * g.field;
+    const E* src = in + fb;
+    E* dst = out + fb;
+    const int N = g.n, L = g.l, P = g.p, B = g.b;
+    const int local = (int)g.local;
+    // block-cyclic global index -> (rank coordinate, local index)
+    auto split = [&](int gi, int& r, int& li) {
+        gi &= N - 1;
+        r = (gi >> g.lb) & (P - 1);
+        li = ((gi >> (g.lb + g.lp)) << g.lb) + (gi & (B - 1));
+    };
+    const bool d3 = g.dims == 3;
+    // base: offset of the thread's line at march position 0 of its axis
+    int base, jj = 0;
+    const int p0 = s * T - 2;  // march position of ring index 0
+    if constexpr (AXIS == 0) {
+        jj = o & (g.vec - 1);
+        const int gr = (o >> g.lvec) & (N - 1);
+        const int gd = d3 ? (o >> (g.lvec + g.ln)) : 0;
+        int rid, lr, did = 0, ldp = 0;
+        split(gr, rid, lr);
+        if (d3) split(gd, did, ldp);
+        base = ((did * P + rid) * P) * local + (ldp * L + lr) * L;
+    } else if constexpr (AXIS == 1) {
+        const int lc = o & (L - 1);
+        const int cid = (o >> g.ll) & (P - 1);
+        const int gd = d3 ? (o >> (g.ll + g.lp)) : 0;
+        int did = 0, ldp = 0;
+        if (d3) split(gd, did, ldp);
+        base = (did * P * P + cid) * local + ldp * L * L + lc;
+    } else {
+        const int lc = o & (L - 1);
+        const int cid = (o >> g.ll) & (P - 1);
+        const int gr = o >> (g.ll + g.lp);
+        int rid, lr;
+        split(gr, rid, lr);
+        base = (rid * P + cid) * local + lr * L + lc;
+    }
+    // offset of ring index i (march position p0 + i)
+    auto at = [&](int i) -> int {
+        const int q = p0 + i;
+        if constexpr (AXIS == 0) {
+            const int bp = B * P;
+            if (q >= 0 && q < bp) return base + (q >> g.lb) * local + (q & (B - 1)) * g.vec + jj;
+            // past an end of the lane: the global column jj B P + q wraps
+            // onto the neighbouring lane (periodic in x)
+            const int gx = (jj * bp + q) & (N - 1);
+            const int q2 = gx & (bp - 1);
+            return base + (q2 >> g.lb) * local + (q2 & (B - 1)) * g.vec + (gx >> (g.lb + g.lp));
+        } else if constexpr (AXIS == 1) {
+            int rid, lr;
+            split(q, rid, lr);
+            return base + rid * P * local + lr * L;
+        } else {
+            int did, ldp;
+            split(q, did, ldp);
+            return base + did * P * P * local + ldp * L * L;
+        }
+    };
+    E w[RING];
+#pragma unroll
+    for (int i = 0; i < AHEAD; i++)
+        if (i < T + 4) w[i] = src[at(i)];
+    for (int t0 = 0; t0 < T; t0 += RING) {
+#pragma unroll
+        for (int u = 0; u < RING; u++) {
+            const int t = t0 + u;
+            if (t < T) {
+                if (t + AHEAD < T + 4) w[(u + AHEAD) & (RING - 1)] = src[at(t + AHEAD)];
+                // output position p0 + t + 2 needs ring indices t .. t + 4
+                const E r = deriv4_combine<O, E, S>(w[(u + 1) & (RING - 1)], w[u & (RING - 1)],
+                                                    w[(u + 3) & (RING - 1)], w[(u + 4) & (RING - 1)], denom,
+                                                    negate != 0);
+                st_nt(dst + at(t + 2), r);
+            }
+        }
+    }
+}
+
+template <typename E, typename S>
+static hipError_t launch_deriv4(const E* in, E* out, const Geo& g, int batch, int axis, S denom, bool negate,
+                                hipStream_t stream) {
+    const int planes = g.dims == 3 ? g.n : 1;
+    // lines of the march axis, and the march length each covers
+    int lines, len;
+    if (axis == 0) {
+        lines = planes * g.n * g.vec;
+        len = g.b * g.p;
+    } else if (axis == 1) {
+        lines = planes * g.n;
+        len = g.n;
+    } else {
+        lines = g.n * g.n;
+        len = g.n;
+    }
+    const int T = len < 64 ? len : 64;
+    int lstrips = 0;
+    while ((T << lstrips) < len) lstrips++;
+    const unsigned blocks = (unsigned)(((int64_t)lines + 255) / 256) << lstrips;
+    const dim3 grid(blocks, (unsigned)batch);
+    const int neg = negate ? 1 : 0;
+    if (axis == 0)
+        hipLaunchKernelGGL((deriv4_march_kernel<E, S, 0>), grid, dim3(256), 0, stream, in, out, g, T, lstrips, lines,
+                           denom, neg);
+    else if (axis == 1)
+        hipLaunchKernelGGL((deriv4_march_kernel<E, S, 1>), grid, dim3(256), 0, stream, in, out, g, T, lstrips, lines,
+                           denom, neg);
+    else
+        hipLaunchKernelGGL((deriv4_march_kernel<E, S, 2>), grid, dim3(256), 0, stream, in, out, g, T, lstrips, lines,
+                           denom, neg);
+    return hipGetLastError();
 }
 
 // 5/7-point Laplacian as a rolling strip: a workgroup owns one column rank
@@ -437,15 +554,9 @@ static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int 
     } else if (op == OP_LAPLACE) {
         hipLaunchKernelGGL((laplace_kernel<E, S>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
                            coef != 1.0 ? 1 : 0);
-    } else if (op == OP_DX4) {
-        hipLaunchKernelGGL((deriv4_kernel<E, S, 0>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
-                           negate ? 1 : 0);
-    } else if (op == OP_DY4) {
-        hipLaunchKernelGGL((deriv4_kernel<E, S, 1>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
-                           negate ? 1 : 0);
     } else {
-        hipLaunchKernelGGL((deriv4_kernel<E, S, 2>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,
-                           negate ? 1 : 0);
+        const int axis = op == OP_DX4 ? 0 : op == OP_DY4 ? 1 : 2;
+        return launch_deriv4<E, S>(src, dst, g, batch, axis, (S)coef, negate, stream);
     }
     return hipGetLastError();
 }
