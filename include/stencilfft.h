@@ -44,7 +44,8 @@ typedef enum {
     ZM_ERR_ALIAS = 5,       /* in and out alias where that is not allowed */
     ZM_ERR_LAUNCH = 6,      /* a kernel launch failed */
     ZM_ERR_NOMEM = 7,       /* device allocation failed */
-    ZM_ERR_IO = 8           /* file I/O failed (stencilfft_hdf5.h) */
+    ZM_ERR_IO = 8,          /* file I/O failed (stencilfft_hdf5.h) */
+    ZM_ERR_SCRATCH_LIMIT = 9 /* the plan needs more scratch than ZM_NOALLOC_SCRATCH_BYTES (zm_forward_noalloc) */
 } zm_status;
 
 typedef enum {
@@ -129,6 +130,25 @@ size_t zm_field_bytes(const zm_plan* plan);
  * checked here), or is NULL to use the buffer the plan allocated at creation
  * (ZM_ERR_INVALID_ARG for a ZM_FLAG_CALLER_SCRATCH plan). */
 zm_status zm_forward(zm_plan* plan, const void* in, void* out, void* scratch, void* stream);
+
+/* Scratch bytes available to zm_forward_noalloc on each GPU. */
+#define ZM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* zm_forward without any device allocation: the same kernels, with the
+ * library's static device scratch (ZM_NOALLOC_SCRATCH_BYTES on each GPU the
+ * library runs on, held from the first zm_plan_create on that GPU until the
+ * process ends) in place of a caller's or the plan's buffer.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc); it calls no hipMalloc, hipFree or allocating library routine,
+ * and records no per-launch timing events.  A plan whose zm_scratch_bytes
+ * exceeds ZM_NOALLOC_SCRATCH_BYTES gets ZM_ERR_SCRATCH_LIMIT before anything
+ * is launched (2D complex64 up to N = 2048 fits, complex128 up to N = 2048,
+ * 3D complex64 up to N = 128).  Every call on one GPU shares that GPU's
+ * buffer, which is fully overwritten before it is read (nothing in it needs
+ * clearing) and keeps the last call's contents: calls that overlap on one GPU
+ * (two streams, two threads) race, so order them on one stream. */
+zm_status zm_forward_noalloc(zm_plan* plan, const void* in, void* out, void* stream);
 
 /* Inverse (unnormalised inverse DFTs, `scale` applied once), reciprocal ->
  * real space.  Replaces inverse_fft<true> (fft.h:40-42, fft.cu:409-514) and

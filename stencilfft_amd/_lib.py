@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(HERE, "libstencilfft.so")
 
 ZM_OK = 0
 (ZM_ERR_INVALID_ARG, ZM_ERR_SHAPE, ZM_ERR_UNSUPPORTED, ZM_ERR_DEVICE, ZM_ERR_ALIAS, ZM_ERR_LAUNCH,
- ZM_ERR_NOMEM, ZM_ERR_IO) = range(1, 9)
+ ZM_ERR_NOMEM, ZM_ERR_IO, ZM_ERR_SCRATCH_LIMIT) = range(1, 10)
+ZM_NOALLOC_SCRATCH_BYTES = 64 << 20
 STATUS_NAMES = {
     0: "ZM_OK",
     1: "ZM_ERR_INVALID_ARG",
@@ -24,6 +25,7 @@ STATUS_NAMES = {
     6: "ZM_ERR_LAUNCH",
     7: "ZM_ERR_NOMEM",
     8: "ZM_ERR_IO",
+    9: "ZM_ERR_SCRATCH_LIMIT",
 }
 
 ZM_FP64, ZM_FP32, ZM_FP32_MFMA = 0, 1, 2
@@ -71,6 +73,7 @@ SIGNATURES = {
     "zm_scratch_bytes": (ctypes.c_size_t, [_VP]),
     "zm_field_bytes": (ctypes.c_size_t, [_VP]),
     "zm_forward": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "zm_forward_noalloc": (_I, [_VP, _VP, _VP, _VP]),
     "zm_inverse": (_I, [_VP, _VP, _VP, _D, _VP, _VP]),
     "zm_permute_forward": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "zm_permute_inverse": (_I, [_VP, _VP, _VP, _VP, _VP]),

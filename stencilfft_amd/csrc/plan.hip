@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -56,8 +57,8 @@ namespace {
 // a stream) to its own: one event per launch, so back-to-back launches are
 // timed without doubling the events in the queue.
 template <typename F>
-hipError_t timed_launch(zm_plan* p, int stage, hipStream_t stream, F&& launch) {
-    if (!p->timing) return launch();
+hipError_t timed_launch(zm_plan* p, int stage, hipStream_t stream, F&& launch, bool record = true) {
+    if (!p->timing || !record) return launch();
     zm_plan::Timed t{stage, nullptr, nullptr, false};
     hipError_t e = hipSuccess;
     if (p->last_stop && p->last_stream == stream) {
@@ -92,6 +93,41 @@ void drop_timing(zm_plan* p) {
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of zm_forward_noalloc: a file-scope device array, one copy on every
+// GPU the library runs on, resident until the process ends.  In host code the
+// name `scratch` is only a stand-in for the symbol, never a device address:
+// zm_plan_create looks up the current device's copy once with
+// hipGetSymbolAddress, and every use goes through that per-device address.
+// ---------------------------------------------------------------------------
+namespace noalloc {
+
+static __device__ __align__(256) unsigned char scratch[ZM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex mu;
+bool looked_up[kMaxDevices];
+void* addr[kMaxDevices];  // this device's copy of `scratch`, or null if the lookup failed
+
+// Called by zm_plan_create with `device` current.
+void resolve(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(mu);
+    if (looked_up[device]) return;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) ptr = nullptr;
+    addr[device] = ptr;
+    looked_up[device] = true;
+}
+
+void* device_copy(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    return looked_up[device] ? addr[device] : nullptr;
+}
+
+}  // namespace noalloc
 
 namespace {
 
@@ -194,6 +230,8 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
     if (hipGetDevice(&device) != hipSuccess) return ZM_ERR_DEVICE;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return ZM_ERR_DEVICE;
+
+    noalloc::resolve(device);
 
     zm_plan* p = new (std::nothrow) zm_plan();
     if (!p) return ZM_ERR_NOMEM;
@@ -355,6 +393,7 @@ struct Exec {
     hipStream_t st;
     bool compute;
     const zm::Cx<T>* tw;
+    bool record = true;  // per-launch timing events when the plan records them
 
     const zm::Cx<T>* line_tables() const {
         return p->precision == ZM_FP64 ? (const zm::Cx<T>*)p->ltab64 : (const zm::Cx<T>*)p->ltab32;
@@ -366,12 +405,12 @@ struct Exec {
                 return zm::launch_line_fast<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, a, line_tables(),
                                                p->batch, fwd, scale, fwd ? p->fast_grid_fwd : p->fast_grid_inv,
                                                p->precision == ZM_FP32_MFMA, st);
-            }));
+            }, record));
         }
         return launch_status(timed_launch(p, ZM_STAGE_LINE, st, [&] {
             return zm::launch_line<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, a, tw, p->batch, fwd, compute,
                                       scale, st);
-        }));
+        }, record));
     }
     zm::StageArgs sargs(int axis) const {
         return zm::StageArgs{axis_stride(p, axis), (int)(p->dims.local / p->dims.l), p->rpb,
@@ -381,19 +420,19 @@ struct Exec {
         return launch_status(timed_launch(p, ZM_STAGE_A, st, [&] {
             return zm::launch_stage_a<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, sargs(axis), tw, p->batch,
                                          fwd, compute, scale, st);
-        }));
+        }, record));
     }
     zm_status stage_b(const void* in, void* out, int axis, bool fwd) const {
         return launch_status(timed_launch(p, ZM_STAGE_B, st, [&] {
             return zm::launch_stage_b<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, sargs(axis), tw, p->batch,
                                          fwd, compute, st);
-        }));
+        }, record));
     }
     zm_status repack(const void* in, void* out, RepackKind k, bool fwd) const {
         return launch_status(timed_launch(p, ZM_STAGE_REPACK, st, [&] {
             return zm::launch_repack<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, repack_args(p, k), p->batch,
                                         fwd, st);
-        }));
+        }, record));
     }
 
     // forward_fft / forward_fft_3d (zmodel/transforms/fft.cu:199-339,
@@ -481,17 +520,17 @@ zm_status scratch_for(const zm_plan* p, void* scratch, void** s) {
 }
 
 zm_status run_transform(zm_plan* p, const void* in, void* out, void* scratch, void* stream, bool fwd, bool compute,
-                        double scale) {
+                        double scale, bool record = true) {
     ZM_TRY(check_io(p, in, out));
     void* s = nullptr;
     ZM_TRY(scratch_for(p, scratch, &s));
     if (s == in || s == out) return ZM_ERR_ALIAS;
     hipStream_t st = (hipStream_t)stream;
     if (p->precision == ZM_FP64) {
-        Exec<double> ex{p, st, compute, (const double2*)p->tw64};
+        Exec<double> ex{p, st, compute, (const double2*)p->tw64, record};
         return fwd ? ex.forward(in, out, s) : ex.inverse(in, out, s, scale);
     }
-    Exec<float> ex{p, st, compute, (const float2*)p->tw32};
+    Exec<float> ex{p, st, compute, (const float2*)p->tw32, record};
     return fwd ? ex.forward(in, out, s) : ex.inverse(in, out, s, scale);
 }
 
@@ -501,6 +540,15 @@ extern "C" {
 
 zm_status zm_forward(zm_plan* p, const void* in, void* out, void* scratch, void* stream) {
     return run_transform(p, in, out, scratch, stream, true, true, 1.0);
+}
+
+zm_status zm_forward_noalloc(zm_plan* p, const void* in, void* out, void* stream) {
+    if (!p) return ZM_ERR_INVALID_ARG;
+    if (p->scratch_bytes > ZM_NOALLOC_SCRATCH_BYTES) return ZM_ERR_SCRATCH_LIMIT;
+    void* s = noalloc::device_copy(p->device);
+    if (!s) return ZM_ERR_DEVICE;
+    // no timing events: creating them is an allocation
+    return run_transform(p, in, out, s, stream, true, true, 1.0, false);
 }
 
 zm_status zm_inverse(zm_plan* p, const void* in, void* out, double scale, void* scratch, void* stream) {
@@ -773,6 +821,7 @@ const char* zm_strerror(zm_status s) {
         case ZM_ERR_LAUNCH: return "kernel launch failed";
         case ZM_ERR_NOMEM: return "device allocation failed";
         case ZM_ERR_IO: return "file I/O failed";
+        case ZM_ERR_SCRATCH_LIMIT: return "the plan needs more scratch than the static no-allocation buffer holds";
     }
     return "unknown status";
 }
