@@ -208,8 +208,9 @@ __device__ __forceinline__ E deriv4_combine(E fm1, E fm2, E fp1, E fp2, S denom,
 
 // 4th-order derivative as a march along the derivative's global axis.  AXIS
 // 0 = x (packed columns), 1 = y (block-cyclic rows), 2 = z (block-cyclic
-// depth, 3D only; the reference has no z derivative).  Every thread owns one
-// line of that axis and walks `T` consecutive global positions of it:
+// depth, 3D only; the reference has no z derivative).  Every thread owns K
+// lines of that axis (256 apart in line index) and walks `T` consecutive
+// global positions of them:
 //   x: one packed lane jj of one global row; position q in [0, B P) is vector
 //      q mod B of column rank q / B, so the march crosses the P ranks of the
 //      row, and the halos before q = 0 and after q = B P - 1 are the
@@ -219,23 +220,24 @@ __device__ __forceinline__ E deriv4_combine(E fm1, E fm2, E fp1, E fp2, S denom,
 //      global rows, each mapped to its row rank and local row (dy.cu:34-97);
 //   z: one (global row, column rank, lc) pencil over global planes.
 // Consecutive threads own consecutive packed columns (lanes for x), so every
-// load and store of a wave is one contiguous run.  The line sits in a
-// 16-slot register ring that runs 12 positions ahead of the output: each
-// value is loaded once per strip (plus 4 halo loads at the strip ends), 8
-// loads are in flight while one output is computed, and no value is ever
-// exchanged between threads.  Strips of one line are consecutive blocks in
-// XCD-aware order, so a strip's halo loads hit the L2 that its neighbour
-// strip's loads filled.
-template <typename E, typename S, int AXIS>
+// load and store of a wave is one contiguous run.  The lines sit in a
+// RING-slot register ring that runs AHEAD positions ahead of the output: each
+// value is loaded once per strip (plus 4 halo loads at the strip ends),
+// (AHEAD - 4) K loads are in flight while one position is computed, and no
+// value is exchanged between threads.  Strips of one line are consecutive
+// blocks in XCD-aware order, so a strip's halo loads hit the L2 that its
+// neighbour strip's loads filled.  NTL: non-temporal loads.
+template <typename E, typename S, int AXIS, int K, int RING, int AHEAD, bool NTL>
 __global__ void __launch_bounds__(256) deriv4_march_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g,
                                                            int T, int lstrips, int lines, S denom, int negate) {
 #pragma clang fp contract(off)
     using O = EOps<E>;
-    constexpr int RING = 16, AHEAD = 12;  // AHEAD < RING: the slot loaded holds a dead value
+    static_assert(AHEAD < RING && AHEAD > 4, "the slot loaded must hold a dead value");
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     const int s = (int)(blk & ((1u << lstrips) - 1));
-    const int o = (int)(blk >> lstrips) * 256 + (int)threadIdx.x;
-    if (o >= lines) return;
+    const int o0 = (int)(blk >> lstrips) * 256 * K + (int)threadIdx.x;
+    // the launch gives K > 1 only when 256 K divides `lines`
+    if (o0 >= lines) return;
     const int64_t fb = (int64_t)blockIdx.y * g.field;
     const E* src = in + fb;
     E* dst = out + fb;
@@ -248,105 +250,134 @@ __global__ void __launch_bounds__(256) deriv4_march_kernel(const E* __restrict__
         li = ((gi >> (g.lb + g.lp)) << g.lb) + (gi & (B - 1));
     };
     const bool d3 = g.dims == 3;
-    // base: offset of the thread's line at march position 0 of its axis
-    int base, jj = 0;
+    // base: offset of each line at march position 0 of its axis
+    int base[K], jj[K];
     const int p0 = s * T - 2;  // march position of ring index 0
-    if constexpr (AXIS == 0) {
-        jj = o & (g.vec - 1);
-        const int gr = (o >> g.lvec) & (N - 1);
-        const int gd = d3 ? (o >> (g.lvec + g.ln)) : 0;
-        int rid, lr, did = 0, ldp = 0;
-        split(gr, rid, lr);
-        if (d3) split(gd, did, ldp);
-        base = ((did * P + rid) * P) * local + (ldp * L + lr) * L;
-    } else if constexpr (AXIS == 1) {
-        const int lc = o & (L - 1);
-        const int cid = (o >> g.ll) & (P - 1);
-        const int gd = d3 ? (o >> (g.ll + g.lp)) : 0;
-        int did = 0, ldp = 0;
-        if (d3) split(gd, did, ldp);
-        base = (did * P * P + cid) * local + ldp * L * L + lc;
-    } else {
-        const int lc = o & (L - 1);
-        const int cid = (o >> g.ll) & (P - 1);
-        const int gr = o >> (g.ll + g.lp);
-        int rid, lr;
-        split(gr, rid, lr);
-        base = (rid * P + cid) * local + lr * L + lc;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int o = o0 + 256 * k;
+        jj[k] = 0;
+        if constexpr (AXIS == 0) {
+            jj[k] = o & (g.vec - 1);
+            const int gr = (o >> g.lvec) & (N - 1);
+            const int gd = d3 ? (o >> (g.lvec + g.ln)) : 0;
+            int rid, lr, did = 0, ldp = 0;
+            split(gr, rid, lr);
+            if (d3) split(gd, did, ldp);
+            base[k] = ((did * P + rid) * P) * local + (ldp * L + lr) * L;
+        } else if constexpr (AXIS == 1) {
+            const int lc = o & (L - 1);
+            const int cid = (o >> g.ll) & (P - 1);
+            const int gd = d3 ? (o >> (g.ll + g.lp)) : 0;
+            int did = 0, ldp = 0;
+            if (d3) split(gd, did, ldp);
+            base[k] = (did * P * P + cid) * local + ldp * L * L + lc;
+        } else {
+            const int lc = o & (L - 1);
+            const int cid = (o >> g.ll) & (P - 1);
+            const int gr = o >> (g.ll + g.lp);
+            int rid, lr;
+            split(gr, rid, lr);
+            base[k] = (rid * P + cid) * local + lr * L + lc;
+        }
     }
-    // offset of ring index i (march position p0 + i)
-    auto at = [&](int i) -> int {
+    // offset of ring index i (march position p0 + i) of line k
+    auto at = [&](int i, int k) -> int {
         const int q = p0 + i;
         if constexpr (AXIS == 0) {
             const int bp = B * P;
-            if (q >= 0 && q < bp) return base + (q >> g.lb) * local + (q & (B - 1)) * g.vec + jj;
+            if (q >= 0 && q < bp) return base[k] + (q >> g.lb) * local + (q & (B - 1)) * g.vec + jj[k];
             // past an end of the lane: the global column jj B P + q wraps
             // onto the neighbouring lane (periodic in x)
-            const int gx = (jj * bp + q) & (N - 1);
+            const int gx = (jj[k] * bp + q) & (N - 1);
             const int q2 = gx & (bp - 1);
-            return base + (q2 >> g.lb) * local + (q2 & (B - 1)) * g.vec + (gx >> (g.lb + g.lp));
+            return base[k] + (q2 >> g.lb) * local + (q2 & (B - 1)) * g.vec + (gx >> (g.lb + g.lp));
         } else if constexpr (AXIS == 1) {
             int rid, lr;
             split(q, rid, lr);
-            return base + rid * P * local + lr * L;
+            return base[k] + rid * P * local + lr * L;
         } else {
             int did, ldp;
             split(q, did, ldp);
-            return base + did * P * P * local + ldp * L * L;
+            return base[k] + did * P * P * local + ldp * L * L;
         }
     };
-    E w[RING];
+    auto ld = [](const E* p) -> E {
+        if constexpr (NTL) return ld_nt(p);
+        else return *p;
+    };
+    E w[RING][K];
 #pragma unroll
     for (int i = 0; i < AHEAD; i++)
-        if (i < T + 4) w[i] = src[at(i)];
+        if (i < T + 4)
+#pragma unroll
+            for (int k = 0; k < K; k++) w[i][k] = ld(src + at(i, k));
     for (int t0 = 0; t0 < T; t0 += RING) {
 #pragma unroll
         for (int u = 0; u < RING; u++) {
             const int t = t0 + u;
             if (t < T) {
-                if (t + AHEAD < T + 4) w[(u + AHEAD) & (RING - 1)] = src[at(t + AHEAD)];
+                if (t + AHEAD < T + 4)
+#pragma unroll
+                    for (int k = 0; k < K; k++) w[(u + AHEAD) & (RING - 1)][k] = ld(src + at(t + AHEAD, k));
                 // output position p0 + t + 2 needs ring indices t .. t + 4
-                const E r = deriv4_combine<O, E, S>(w[(u + 1) & (RING - 1)], w[u & (RING - 1)],
-                                                    w[(u + 3) & (RING - 1)], w[(u + 4) & (RING - 1)], denom,
-                                                    negate != 0);
-                st_nt(dst + at(t + 2), r);
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const E r = deriv4_combine<O, E, S>(w[(u + 1) & (RING - 1)][k], w[u & (RING - 1)][k],
+                                                        w[(u + 3) & (RING - 1)][k], w[(u + 4) & (RING - 1)][k],
+                                                        denom, negate != 0);
+                    st_nt(dst + at(t + 2, k), r);
+                }
             }
         }
     }
 }
 
+// Launch shapes, from a sweep of (K, RING, AHEAD, T, load hint) at 8192^2
+// P=4 B=32 and 512^3 P=2 B=16 on the MI355X (complex FP32 unless noted):
+//   x: K=1, ring 16 / 12 ahead, T = min(B P, 128), non-temporal loads in 2D
+//      (2D 0.195 ms, 3D 0.411 ms);
+//   y: 2D K=2, ring 8 / 6 ahead, T=64, non-temporal loads (0.203 ms; FP64
+//      K=2, ring 16 / 12 ahead, T=32: 0.408 ms); 3D K=1, ring 16 / 12, T=32
+//      (0.424 ms);
+//   z: K=4, ring 8 / 6 ahead (FP64 4 ahead), T=128 (0.420 ms; FP64 0.819 ms).
+// A K that does not divide the line count falls back to K=1.
+template <typename E, typename S, int AXIS, int K, int RING, int AHEAD, bool NTL>
+static hipError_t launch_march(const E* in, E* out, const Geo& g, int batch, int lines, int len, int tmax, S denom,
+                               bool negate, hipStream_t stream) {
+    if constexpr (K > 1) {
+        if (lines % (256 * K))
+            return launch_march<E, S, AXIS, 1, 16, 12, NTL>(in, out, g, batch, lines, len, tmax, denom, negate, stream);
+    }
+    const int T = len < tmax ? len : tmax;
+    int lstrips = 0;
+    while ((T << lstrips) < len) lstrips++;
+    const unsigned blocks = (unsigned)(((int64_t)lines + 256 * K - 1) / (256 * K)) << lstrips;
+    hipLaunchKernelGGL((deriv4_march_kernel<E, S, AXIS, K, RING, AHEAD, NTL>), dim3(blocks, (unsigned)batch),
+                       dim3(256), 0, stream, in, out, g, T, lstrips, lines, denom, negate ? 1 : 0);
+    return hipGetLastError();
+}
+
 template <typename E, typename S>
 static hipError_t launch_deriv4(const E* in, E* out, const Geo& g, int batch, int axis, S denom, bool negate,
                                 hipStream_t stream) {
-    const int planes = g.dims == 3 ? g.n : 1;
-    // lines of the march axis, and the march length each covers
-    int lines, len;
+    const bool d3 = g.dims == 3;
+    const int planes = d3 ? g.n : 1;
+    constexpr bool wide = sizeof(E) >= 16;  // complex FP64
     if (axis == 0) {
-        lines = planes * g.n * g.vec;
-        len = g.b * g.p;
-    } else if (axis == 1) {
-        lines = planes * g.n;
-        len = g.n;
-    } else {
-        lines = g.n * g.n;
-        len = g.n;
+        const int lines = planes * g.n * g.vec, len = g.b * g.p;
+        if (d3) return launch_march<E, S, 0, 1, 16, 12, false>(in, out, g, batch, lines, len, 128, denom, negate, stream);
+        return launch_march<E, S, 0, 1, 16, 12, true>(in, out, g, batch, lines, len, 128, denom, negate, stream);
     }
-    const int T = len < 64 ? len : 64;
-    int lstrips = 0;
-    while ((T << lstrips) < len) lstrips++;
-    const unsigned blocks = (unsigned)(((int64_t)lines + 255) / 256) << lstrips;
-    const dim3 grid(blocks, (unsigned)batch);
-    const int neg = negate ? 1 : 0;
-    if (axis == 0)
-        hipLaunchKernelGGL((deriv4_march_kernel<E, S, 0>), grid, dim3(256), 0, stream, in, out, g, T, lstrips, lines,
-                           denom, neg);
-    else if (axis == 1)
-        hipLaunchKernelGGL((deriv4_march_kernel<E, S, 1>), grid, dim3(256), 0, stream, in, out, g, T, lstrips, lines,
-                           denom, neg);
-    else
-        hipLaunchKernelGGL((deriv4_march_kernel<E, S, 2>), grid, dim3(256), 0, stream, in, out, g, T, lstrips, lines,
-                           denom, neg);
-    return hipGetLastError();
+    if (axis == 1) {
+        const int lines = planes * g.n;
+        if (d3) return launch_march<E, S, 1, 1, 16, 12, false>(in, out, g, batch, lines, g.n, 32, denom, negate, stream);
+        if (wide) return launch_march<E, S, 1, 2, 16, 12, false>(in, out, g, batch, lines, g.n, 32, denom, negate, stream);
+        return launch_march<E, S, 1, 2, 8, 6, true>(in, out, g, batch, lines, g.n, 64, denom, negate, stream);
+    }
+    const int lines = g.n * g.n;
+    if (wide) return launch_march<E, S, 2, 4, 8, 5, false>(in, out, g, batch, lines, g.n, 128, denom, negate, stream);
+    return launch_march<E, S, 2, 4, 8, 6, false>(in, out, g, batch, lines, g.n, 128, denom, negate, stream);
 }
 
 // 5/7-point Laplacian as a rolling strip: a workgroup owns one column rank
