@@ -520,6 +520,144 @@ __global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict_
     }
 }
 
+// 3D 7-point Laplacian as a march along depth over row tiles (2.5D
+// blocking): a workgroup owns one column rank cid, R consecutive global rows
+// and a strip of tz consecutive global planes.  The tile of each plane (R
+// rows of L packed columns) is read once: the planes before, at and after
+// the current one sit in a ring of four register slots (the fourth is the
+// lookahead plane in flight), so the z-neighbours cost no extra reads.  The
+// current plane goes to LDS with one halo row above and below (rows r0 - 1
+// and r0 + R, the only re-read data: 2 / R of a plane) and the VEC-wide halo
+// vectors of the neighbouring column ranks, lane-shifted at the global x edge
+// (laplace.cu:148,160), so above / below / left / right are LDS reads.  Row
+// tiles of one (cid, strip) are consecutive blocks in XCD-aware order, so
+// the halo rows a tile re-reads were just fetched into the same L2 by its
+// neighbour tiles.  Against the row strip kernel, which re-reads the rows of
+// planes gd +- 1 from L2 (FETCH 1.32x the field at 512^3), this took the
+// 512^3 FP32 Laplacian from 0.60 to 0.41 ms; in FP64 the strip kernel stayed
+// ahead (0.93 against 0.95 ms), so FP64 keeps it where it applies.
+// KE = R L / 256 tile elements per thread; KY, HX: y-halo and x-halo elements
+// per thread (guarded).
+template <typename E, typename S, int R, int KE, int KY, int HX>
+__global__ void __launch_bounds__(256) laplace_zmarch_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, int tz,
+                                                   int ltiles, S scale, int apply_scale) {
+#pragma clang fp contract(off)
+    using O = EOps<E>;
+    extern __shared__ unsigned char smem_raw[];
+    E* tile = reinterpret_cast<E*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int N = g.n, L = g.l, VEC = g.vec, B = g.b, P = g.p;
+    const int W = L + 2 * VEC;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int rt = (int)(blk & ((1u << ltiles) - 1));
+    const int rest = (int)(blk >> ltiles);
+    const int cid = rest & (P - 1);
+    const int zs = rest >> g.lp;
+    const int r0 = rt * R, z0 = zs * tz;
+    const int64_t fbase = (int64_t)blockIdx.y * g.field;
+    const E* src = in + fbase;
+    E* dst = out + fbase;
+    const int cl = (cid - 1 + P) & (P - 1), cr = (cid + 1) & (P - 1);
+    auto split = [&](int gi, int& r, int& li) {
+        gi &= N - 1;
+        r = (gi >> g.lb) & (P - 1);
+        li = ((gi >> (g.lb + g.lp)) << g.lb) + (gi & (B - 1));
+    };
+    auto row_off = [&](int z, int gr, int c) -> int {
+        int rid, lr, did, ldp;
+        split(gr, rid, lr);
+        split(z, did, ldp);
+        const int id = (((did << g.lp) + rid) << g.lp) + c;
+        return id * (int)g.local + (ldp * L + lr) * L;
+    };
+    const int nyh = 2 * L, nxh = 2 * R * VEC;
+    auto load_plane = [&](int z, E* c) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < KE; k++) {
+            const int e = tid + 256 * k;
+            c[k] = src[row_off(z, r0 + (e >> g.ll), cid) + (e & (L - 1))];
+        }
+    };
+    auto load_halo = [&](int z, E* yh, E* xh) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < KY; k++) {
+            const int e = tid + 256 * k;
+            if (e < nyh) {
+                const int side = e >> g.ll;  // 0: row r0 - 1, 1: row r0 + R
+                yh[k] = src[row_off(z, side ? r0 + R : r0 - 1, cid) + (e & (L - 1))];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < HX; k++) {
+            const int e = tid + 256 * k;
+            if (e < nxh) {
+                const int row = e >> (g.lvec + 1), side = (e >> g.lvec) & 1, jj = e & (VEC - 1);
+                if (side == 0) {
+                    const int jl = (cid == 0) ? ((jj - 1 + VEC) & (VEC - 1)) : jj;
+                    xh[k] = src[row_off(z, r0 + row, cl) + (B - 1) * VEC + jl];
+                } else {
+                    const int jr = (cid == P - 1) ? ((jj + 1) & (VEC - 1)) : jj;
+                    xh[k] = src[row_off(z, r0 + row, cr) + jr];
+                }
+            }
+        }
+    };
+    E pl[4][KE], yh[2][KY > 0 ? KY : 1], xh[2][HX > 0 ? HX : 1];
+    load_plane(z0 - 1, pl[0]);
+    load_plane(z0, pl[1]);
+    load_plane(z0 + 1, pl[2]);
+    load_halo(z0, yh[0], xh[0]);
+    auto step = [&](int t, auto uc, auto hc) __attribute__((always_inline)) {
+        constexpr int U = decltype(uc)::value, C0 = (U + 1) & 3, D = (U + 2) & 3, X = (U + 3) & 3;
+        constexpr int H = decltype(hc)::value, HN = H ^ 1;
+        const int gd = z0 + t;
+        if (t + 1 < tz) {
+            load_plane(gd + 2, pl[X]);
+            load_halo(gd + 1, yh[HN], xh[HN]);
+        }
+#pragma unroll
+        for (int k = 0; k < KE; k++) {
+            const int e = tid + 256 * k;
+            tile[((e >> g.ll) + 1) * W + VEC + (e & (L - 1))] = pl[C0][k];
+        }
+#pragma unroll
+        for (int k = 0; k < KY; k++) {
+            const int e = tid + 256 * k;
+            if (e < nyh) tile[(e >> g.ll) * (R + 1) * W + VEC + (e & (L - 1))] = yh[H][k];
+        }
+#pragma unroll
+        for (int k = 0; k < HX; k++) {
+            const int e = tid + 256 * k;
+            if (e < nxh) {
+                const int row = e >> (g.lvec + 1), side = (e >> g.lvec) & 1, jj = e & (VEC - 1);
+                tile[(row + 1) * W + (side ? VEC + L : 0) + jj] = xh[H][k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KE; k++) {
+            const int e = tid + 256 * k;
+            const int r = (e >> g.ll) + 1, c = e & (L - 1);
+            const int pos = r * W + VEC + c;
+            const E a = tile[pos - W], b = tile[pos + W], l = tile[pos - VEC], rr = tile[pos + VEC];
+            E acc = lap_combine<sizeof(S) == 8, O, E, S>(a, b, l, rr, pl[U][k], pl[D][k], pl[C0][k], true);
+            if (apply_scale) acc = O::mul(acc, scale);
+            dst[row_off(gd, r0 + r - 1, cid) + c] = acc;
+        }
+        __syncthreads();
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    for (int t = 0; t < tz; t += 4) {
+        step(t, I0{}, I0{});
+        step(t + 1, I1{}, I1{});
+        step(t + 2, I2{}, I0{});
+        step(t + 3, I3{}, I1{});
+    }
+}
+
 // Rows per strip: 64 (each row read once, two halo rows per strip), fewer
 // for small 2D fields so that the grid still has about 512 workgroups.
 static int strip_rows(const Geo& g, int batch) {
@@ -573,6 +711,55 @@ static bool strip_ok(const Geo& g, int batch) {
     return lok && (int64_t)batch * g.n <= 65535 && (strips * g.p) % 8 == 0;
 }
 
+// Row tile height of the depth march: 8 rows, fewer for long rows so that a
+// thread holds at most 8 tile elements per plane.
+static int zmarch_rows(const Geo& g) { return g.l <= 256 ? 8 : 2048 / g.l; }
+
+// The depth march covers 3D shapes with 32 <= L <= 1024 whose x halos fit
+// two elements per thread (R VEC <= 256), whose planes split into strips of
+// a multiple of 4, and whose LDS tile (R + 2 rows of L + 2 VEC elements of
+// `es` bytes) fits the default 64 KiB.
+static bool zmarch_ok(const Geo& g, size_t es) {
+    if (g.dims != 3 || g.l < 32 || g.l > 1024 || g.n < 32) return false;
+    const int R = zmarch_rows(g);
+    return R * g.vec <= 256 && g.n % R == 0 && (size_t)(R + 2) * (size_t)(g.l + 2 * g.vec) * es <= 65536;
+}
+
+template <typename E, typename S, int R, int KE, int KY>
+static hipError_t laplace_zmarch_inst(const E* in, E* out, const Geo& g, int batch, double coef, hipStream_t stream) {
+    const int ntiles = g.n / R;
+    int ltiles = 0;
+    while ((1 << ltiles) < ntiles) ltiles++;
+    // depth strips: the longest (up to 128 planes) that still gives 512
+    // workgroups (two per CU); 512^3 FP32 ran 0.41 ms at 128 planes against
+    // 0.45-0.47 ms at 32-64
+    int tz = g.n < 128 ? g.n : 128;
+    while (tz > 32 && (int64_t)ntiles * g.p * (g.n / tz) * batch < 512) tz >>= 1;
+    const unsigned blocks = (unsigned)((int64_t)ntiles * g.p * (g.n / tz));
+    const size_t lds = sizeof(E) * (size_t)(R + 2) * (size_t)(g.l + 2 * g.vec);
+    const bool two = 2 * R * g.vec > 256;
+    if (two)
+        hipLaunchKernelGGL((laplace_zmarch_kernel<E, S, R, KE, KY, 2>), dim3(blocks, (unsigned)batch), dim3(256), lds,
+                           stream, in, out, g, tz, ltiles, (S)coef, coef != 1.0 ? 1 : 0);
+    else
+        hipLaunchKernelGGL((laplace_zmarch_kernel<E, S, R, KE, KY, 1>), dim3(blocks, (unsigned)batch), dim3(256), lds,
+                           stream, in, out, g, tz, ltiles, (S)coef, coef != 1.0 ? 1 : 0);
+    return hipGetLastError();
+}
+
+template <typename E, typename S>
+static hipError_t laplace_zmarch(const E* in, E* out, const Geo& g, int batch, double coef, hipStream_t stream) {
+    switch (g.l) {
+        case 32: return laplace_zmarch_inst<E, S, 8, 1, 1>(in, out, g, batch, coef, stream);
+        case 64: return laplace_zmarch_inst<E, S, 8, 2, 1>(in, out, g, batch, coef, stream);
+        case 128: return laplace_zmarch_inst<E, S, 8, 4, 1>(in, out, g, batch, coef, stream);
+        case 256: return laplace_zmarch_inst<E, S, 8, 8, 2>(in, out, g, batch, coef, stream);
+        case 512: return laplace_zmarch_inst<E, S, 4, 8, 4>(in, out, g, batch, coef, stream);
+        case 1024: return laplace_zmarch_inst<E, S, 2, 8, 8>(in, out, g, batch, coef, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <typename E, typename S>
 static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int batch, StencilOp op, double coef,
                                    bool negate, hipStream_t stream) {
@@ -580,7 +767,12 @@ static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int 
     const dim3 grid((unsigned)((g.field + threads - 1) / threads), (unsigned)batch);
     const E* src = static_cast<const E*>(in);
     E* dst = static_cast<E*>(out);
-    if (op == OP_LAPLACE && strip_ok(g, batch)) {
+    // 3D: the depth march, except FP64 where the strip kernel applies (there
+    // it measured faster)
+    constexpr bool fp64 = sizeof(S) == 8;
+    if (op == OP_LAPLACE && zmarch_ok(g, sizeof(E)) && !(fp64 && strip_ok(g, batch))) {
+        return laplace_zmarch<E, S>(src, dst, g, batch, coef, stream);
+    } else if (op == OP_LAPLACE && strip_ok(g, batch)) {
         return laplace_strip<E, S>(src, dst, g, batch, coef, stream);
     } else if (op == OP_LAPLACE) {
         hipLaunchKernelGGL((laplace_kernel<E, S>), grid, dim3(threads), 0, stream, src, dst, g, (S)coef,

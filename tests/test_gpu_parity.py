@@ -242,6 +242,34 @@ def test_laplace_fp64_bit_exact(oracle_mod, dims, n, p, b):
     assert np.allclose(gotr, s.laplace(xr) * n * n, rtol=1e-15, atol=0)
 
 
+# 3D shapes of the depth-march Laplacian (row tiles of 8 rows marching over
+# depth): L = 32, 64 and 128, one and two x-halo elements per thread, P = 1
+# and 2, and the 256^3 P=2 B=8 shape of the batched heat-stepping workload.
+DEPTH_MARCH_SHAPES = [(64, 2, 4), (64, 2, 2), (64, 1, 2), (128, 2, 8), (128, 2, 2), (256, 2, 8)]
+
+
+@pytest.mark.parametrize("n,p,b", DEPTH_MARCH_SHAPES)
+def test_laplace_3d_depth_march(oracle_mod, n, p, b):
+    """FP64 bit for bit against the oracle (laplace3d.cu:90-225), complex and
+    real, a batch of two; FP32 within 1e-6 relative L2."""
+    note(f"depth-march laplace 3D N={n} P={p} B={b}")
+    s = oracle_mod.Shape(3, n, p, b)
+    x = s.natural_to_real(uniform_field((n,) * 3, 700 + n + b))
+    x2 = s.natural_to_real(uniform_field((n,) * 3, 800 + n + b))
+    want, want2 = s.laplace(x), s.laplace(x2)
+    pb = plan(3, n, p, b, "fp64", batch=2)
+    got = host(pb.laplace(dev(np.concatenate([x, x2]), torch.complex128)))
+    assert np.array_equal(got[:x.size].view(np.int64), want.view(np.int64))
+    assert np.array_equal(got[x.size:].view(np.int64), want2.view(np.int64))
+    pl = plan(3, n, p, b, "fp64")
+    xr = np.ascontiguousarray(x.real)
+    gotr = host(pl.laplace(dev(xr, torch.float64), scale=float(n * n)))
+    assert np.array_equal(gotr, s.laplace(xr) * float(n * n))
+    p32 = plan(3, n, p, b, "fp32")
+    g32 = host(p32.laplace(dev(x.astype(np.complex64), torch.complex64))).astype(np.complex128)
+    assert rel_l2(g32, s.laplace(x.astype(np.complex64).astype(np.complex128))) < 1e-6
+
+
 @pytest.mark.parametrize("n,p,b", [(16, 2, 2), (32, 2, 4), (64, 4, 2), (32, 2, 8), (64, 2, 2), (256, 4, 8)])
 def test_dx_dy_fp64_bit_exact(oracle_mod, n, p, b):
     note(f"dx/dy N={n} P={p} B={b}")
