@@ -333,14 +333,17 @@ __global__ void __launch_bounds__(256) deriv4_march_kernel(const E* __restrict__
     }
 }
 
-// Launch shapes, from a sweep of (K, RING, AHEAD, T, load hint) at 8192^2
-// P=4 B=32 and 512^3 P=2 B=16 on the MI355X (complex FP32 unless noted):
-//   x: K=1, ring 16 / 12 ahead, T = min(B P, 128), non-temporal loads in 2D
-//      (2D 0.195 ms, 3D 0.411 ms);
-//   y: 2D K=2, ring 8 / 6 ahead, T=64, non-temporal loads (0.203 ms; FP64
-//      K=2, ring 16 / 12 ahead, T=32: 0.408 ms); 3D K=1, ring 16 / 12, T=32
-//      (0.424 ms);
-//   z: K=4, ring 8 / 6 ahead (FP64 4 ahead), T=128 (0.420 ms; FP64 0.819 ms).
+// Launch shapes, from sweeps of (K, RING, AHEAD, T, load hint) at 8192^2
+// P=4 B=32 and 512^3 P=2 B=16 on the MI355X (one box, all variants side by
+// side; complex FP32, FP64 where it differs):
+//   x: K=1, ring 16 / 12 ahead; T = 128 in 2D (FP64 32) with non-temporal
+//      loads, 16 in 3D (2D 0.209 ms, 3D 0.401 ms; FP64 0.455, 0.781 ms);
+//   y: 2D K=2 with non-temporal loads, ring 8 / 6 ahead and T = 256 (FP64
+//      ring 16 / 12 ahead, T = 32, plain loads); 3D K=1, ring 16 / 12, T = 16
+//      (FP64 32) (2D 0.192 ms, 3D 0.408 ms; FP64 0.435, 0.827 ms);
+//   z: K=4, ring 8 / 6 ahead (FP64 5), T = 256 (0.419 ms; FP64 0.835 ms).
+// Longer strips re-read fewer halo positions; shorter ones give more
+// workgroups: neither wins everywhere.
 // A K that does not divide the line count falls back to K=1.
 template <typename E, typename S, int AXIS, int K, int RING, int AHEAD, bool NTL>
 static hipError_t launch_march(const E* in, E* out, const Geo& g, int batch, int lines, int len, int tmax, S denom,
@@ -366,18 +369,21 @@ static hipError_t launch_deriv4(const E* in, E* out, const Geo& g, int batch, in
     constexpr bool wide = sizeof(E) >= 16;  // complex FP64
     if (axis == 0) {
         const int lines = planes * g.n * g.vec, len = g.b * g.p;
-        if (d3) return launch_march<E, S, 0, 1, 16, 12, false>(in, out, g, batch, lines, len, 128, denom, negate, stream);
-        return launch_march<E, S, 0, 1, 16, 12, true>(in, out, g, batch, lines, len, 128, denom, negate, stream);
+        if (d3) return launch_march<E, S, 0, 1, 16, 12, false>(in, out, g, batch, lines, len, 16, denom, negate, stream);
+        return launch_march<E, S, 0, 1, 16, 12, true>(in, out, g, batch, lines, len, wide ? 32 : 128, denom, negate,
+                                                      stream);
     }
     if (axis == 1) {
         const int lines = planes * g.n;
-        if (d3) return launch_march<E, S, 1, 1, 16, 12, false>(in, out, g, batch, lines, g.n, 32, denom, negate, stream);
+        if (d3)
+            return launch_march<E, S, 1, 1, 16, 12, false>(in, out, g, batch, lines, g.n, wide ? 32 : 16, denom, negate,
+                                                           stream);
         if (wide) return launch_march<E, S, 1, 2, 16, 12, false>(in, out, g, batch, lines, g.n, 32, denom, negate, stream);
-        return launch_march<E, S, 1, 2, 8, 6, true>(in, out, g, batch, lines, g.n, 64, denom, negate, stream);
+        return launch_march<E, S, 1, 2, 8, 6, true>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
     }
     const int lines = g.n * g.n;
-    if (wide) return launch_march<E, S, 2, 4, 8, 5, false>(in, out, g, batch, lines, g.n, 128, denom, negate, stream);
-    return launch_march<E, S, 2, 4, 8, 6, false>(in, out, g, batch, lines, g.n, 128, denom, negate, stream);
+    if (wide) return launch_march<E, S, 2, 4, 8, 5, false>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
+    return launch_march<E, S, 2, 4, 8, 6, false>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
 }
 
 // 5/7-point Laplacian as a rolling strip: a workgroup owns one column rank
@@ -529,10 +535,10 @@ __global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict_
 // current plane goes to LDS with one halo row above and below (rows r0 - 1
 // and r0 + R, the only re-read data: 2 / R of a plane) and the VEC-wide halo
 // vectors of the neighbouring column ranks, lane-shifted at the global x edge
-// (laplace.cu:148,160), so above / below / left / right are LDS reads.  Row
-// tiles of one (cid, strip) are consecutive blocks in XCD-aware order, so
-// the halo rows a tile re-reads were just fetched into the same L2 by its
-// neighbour tiles.  Against the row strip kernel, which re-reads the rows of
+// (laplace.cu:148,160), so above / below / left / right are LDS reads.  The
+// tiles whose rows a tile re-reads as halos (row tiles -+ 1, column ranks
+// cid -+ 1) are near it in XCD-aware block order, so those rows were just
+// fetched into the same L2.  Against the row strip kernel, which re-reads the rows of
 // planes gd +- 1 from L2 (FETCH 1.32x the field at 512^3), this took the
 // 512^3 FP32 Laplacian from 0.60 to 0.41 ms; in FP64 the strip kernel stayed
 // ahead (0.93 against 0.95 ms), so FP64 keeps it where it applies.
@@ -548,11 +554,13 @@ __global__ void __launch_bounds__(256) laplace_zmarch_kernel(const E* __restrict
     const int tid = threadIdx.x;
     const int N = g.n, L = g.l, VEC = g.vec, B = g.b, P = g.p;
     const int W = L + 2 * VEC;
+    // logical block = (depth strip, row tile, cid), cid fastest: in XCD-aware
+    // order the tiles that trade x halos (cid -+ 1) and y halo rows (row
+    // tiles -+ 1) run on one XCD at the same time
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-    const int rt = (int)(blk & ((1u << ltiles) - 1));
-    const int rest = (int)(blk >> ltiles);
-    const int cid = rest & (P - 1);
-    const int zs = rest >> g.lp;
+    const int cid = (int)(blk & (unsigned)(P - 1));
+    const int rt = (int)((blk >> g.lp) & ((1u << ltiles) - 1));
+    const int zs = (int)(blk >> (g.lp + ltiles));
     const int r0 = rt * R, z0 = zs * tz;
     const int64_t fbase = (int64_t)blockIdx.y * g.field;
     const E* src = in + fbase;
