@@ -131,6 +131,20 @@ size_t zm_field_bytes(const zm_plan* plan);
  * (ZM_ERR_INVALID_ARG for a ZM_FLAG_CALLER_SCRATCH plan). */
 zm_status zm_forward(zm_plan* plan, const void* in, void* out, void* scratch, void* stream);
 
+/* Forward transform of `scale` times the periodic 5-point (2D) / 7-point (3D)
+ * Laplacian of `in`: the reference's laplace followed by forward_fft
+ * (zmodel/main.cu:50-63, laplace.cu:73-169 / laplace3d.cu:90-225, fft.cu:
+ * 199-339), in one chain of three (2D) or five (3D) passes instead of four or
+ * six.  The directions before the last transform `in` itself; the last
+ * direction's line pass forms the Laplacian of what it reads, multiplying by
+ * the exact symbol -4 sin^2(pi k / N) along the transformed directions and
+ * taking the second difference along its own, then transforms it.  Results
+ * equal zm_forward(zm_laplace(in)) up to rounding (not bit for bit).  Plans
+ * without ZM_PATH_FAST_LINE run the Laplacian and then the forward chain
+ * (four / six passes); ZM_PATH_TWO_STAGE plans get ZM_ERR_UNSUPPORTED.
+ * Buffers as zm_forward. */
+zm_status zm_laplace_forward(zm_plan* plan, const void* in, void* out, double scale, void* scratch, void* stream);
+
 /* Scratch bytes available to zm_forward_noalloc on each GPU. */
 #define ZM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
 

@@ -74,6 +74,7 @@ SIGNATURES = {
     "zm_field_bytes": (ctypes.c_size_t, [_VP]),
     "zm_forward": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "zm_forward_noalloc": (_I, [_VP, _VP, _VP, _VP]),
+    "zm_laplace_forward": (_I, [_VP, _VP, _VP, _D, _VP, _VP]),
     "zm_inverse": (_I, [_VP, _VP, _VP, _D, _VP, _VP]),
     "zm_permute_forward": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "zm_permute_inverse": (_I, [_VP, _VP, _VP, _VP, _VP]),

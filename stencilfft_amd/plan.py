@@ -233,6 +233,16 @@ class Plan:
                 "zm_inverse")
         return out
 
+    def laplace_forward(self, x, scale: float = 1.0, out=None, scratch=None, stream=None):
+        """Forward transform of scale * Laplacian(x) in one chain (the Laplacian
+        + forward_fft of zmodel/main.cu:50-63; equal to forward(laplace(x))
+        up to rounding)."""
+        out = self._out(x, out)
+        sp = self._scratch(scratch, x, out)
+        L.check(L._lib().zm_laplace_forward(self._h, _ptr(x), _ptr(out), float(scale), sp, _stream(stream)),
+                "zm_laplace_forward")
+        return out
+
     def permute_forward(self, x, out=None, scratch=None, stream=None):
         """forward_fft<false>: the data movement alone."""
         out = self._out(x, out)
