@@ -102,16 +102,39 @@ struct LineStencil {
     T scale;       // the Laplacian's scale (1 / h^2)
 };
 
+template <typename T>
+__device__ __forceinline__ Cx<T> lap_combine_line(Cx<T> a, Cx<T> b, Cx<T> c, T sig, T scale) {
+    const T re = ((a.x - c.x) + (b.x - c.x)) + sig * c.x;
+    const T im = ((a.y - c.y) + (b.y - c.y)) + sig * c.y;
+    return Cx<T>{re * scale, im * scale};
+}
+
+// Element p (LDS index, unpadded) of vector m with 0 < m < M - 1: its line
+// neighbours are p -+ VEC, at a constant distance in the padded index when VEC
+// is a multiple of the padding period.
+template <typename T, int LVEC, int LN>
+__device__ __forceinline__ Cx<T> line_laplacian_inner(const Cx<T>* src, int p, Cx<T> c, T sig, T scale) {
+    constexpr int VEC = 1 << LVEC, PS = kPadShift<T, LN>;
+    Cx<T> a, b;
+    if constexpr ((VEC & ((1 << PS) - 1)) == 0) {
+        constexpr int PV = VEC + (VEC >> PS);
+        const int pp = fidx<T, LN>(p);
+        a = src[pp - PV];
+        b = src[pp + PV];
+    } else {
+        a = src[fidx<T, LN>(p - VEC)];
+        b = src[fidx<T, LN>(p + VEC)];
+    }
+    return lap_combine_line<T>(a, b, c, sig, scale);
+}
+
 template <typename T, int LVEC, int LM, int LN>
 __device__ __forceinline__ Cx<T> line_laplacian(const Cx<T>* src, int line, int m, int v, Cx<T> c, T sig, T scale) {
     constexpr int VEC = 1 << LVEC, M = 1 << LM;
     const int lb = line << LN;
     const int ia = m > 0 ? lb + ((m - 1) << LVEC) + v : lb + ((M - 1) << LVEC) + ((v - 1) & (VEC - 1));
     const int ib = m < M - 1 ? lb + ((m + 1) << LVEC) + v : lb + ((v + 1) & (VEC - 1));
-    const Cx<T> a = src[fidx<T, LN>(ia)], b = src[fidx<T, LN>(ib)];
-    const T re = ((a.x - c.x) + (b.x - c.x)) + sig * c.x;
-    const T im = ((a.y - c.y) + (b.y - c.y)) + sig * c.y;
-    return Cx<T>{re * scale, im * scale};
+    return lap_combine_line<T>(src[fidx<T, LN>(ia)], src[fidx<T, LN>(ib)], c, sig, scale);
 }
 
 // One Stockham radix-R pass over the line-group held in LDS.
@@ -172,9 +195,15 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
             constexpr int LM_ = LN - LVEC;
             const int m = bt & ((1 << LM_) - 1), line = bt >> LM_;
             const T sg = stn.sig[line];
+            if (m > 0 && m < (1 << LM_) - 1) {
 #pragma unroll
-            for (int r = 0; r < R; r++)
-                v[k][r] = line_laplacian<T, LVEC, LM_, LN>(src, line, m, j + r * nbf, v[k][r], sg, stn.scale);
+                for (int r = 0; r < R; r++)
+                    v[k][r] = line_laplacian_inner<T, LVEC, LN>(src, base + j + r * nbf, v[k][r], sg, stn.scale);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    v[k][r] = line_laplacian<T, LVEC, LM_, LN>(src, line, m, j + r * nbf, v[k][r], sg, stn.scale);
+            }
         }
         if (PRE) {
 #pragma unroll
@@ -344,7 +373,11 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
                 static_assert(CONTIG && LN_T == LVEC && !PRE, "the line Laplacian precedes the VEC DFTs");
                 constexpr int LM_ = LN - LVEC;
                 const int m = bt & ((1 << LM_) - 1), line = bt >> LM_;
-                x[i] = line_laplacian<T, LVEC, LM_, LN>(src, line, m, j + r * nbf, x[i], stn.sig[line], stn.scale);
+                if (m > 0 && m < (1 << LM_) - 1)
+                    x[i] = line_laplacian_inner<T, LVEC, LN>(src, base + j + r * nbf, x[i], stn.sig[line], stn.scale);
+                else
+                    x[i] = line_laplacian<T, LVEC, LM_, LN>(src, line, m, j + r * nbf, x[i], stn.sig[line],
+                                                            stn.scale);
             }
             if (PRE) {
                 const int e = ((j + r * nbf) * kk) & (N - 1);
