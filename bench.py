@@ -5,7 +5,12 @@ Workload (default): 2D N=8192, P=4 (16 virtual ranks), B=32.  One step is the
 round trip of the reference's example() (zmodel/main.cu:50-80): 5-point
 Laplacian (h = 1/N) -> forward 2D FFT -> inverse 2D FFT scaled 1/N^2, on a
 seeded smooth field (256 random Fourier modes with |k| <= N/8, amplitudes
-N(0,1)/(1+|k|^2), uniform phases, plus 1e-3 uniform noise) held in HBM.
+N(0,1)/(1+|k|^2), uniform phases, plus 1e-3 uniform noise) held in HBM.  The
+step runs as zm_laplace_forward (the Laplacian formed inside the forward
+transform's x pass: 3 kernel sweeps instead of 4, bit for bit the same
+result) then zm_inverse; --separate times the reference's three calls
+instead, and the other chain is always timed beside the headline
+(`other_chain`).
 
 Prints one JSON line on stdout (diagnostics go to stderr):
   value        FFT GFLOP/s, 5 N^2 log2(N^2) per transform, two transforms per
@@ -53,25 +58,35 @@ def rel_errors(a, ref, torch):
     return {"rel_l2": l2, "rel_max": mx}
 
 
-def run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream):
+def run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream, fused=True):
+    """One step = the Laplacian (h = 1/N), the forward transform and the
+    inverse transform (1/N^d).  fused: the Laplacian is formed inside the
+    forward transform's last direction pass (zm_laplace_forward, one chain);
+    otherwise the three calls of the reference's example() (zm_laplace,
+    zm_forward, zm_inverse), with the Laplacian written to `lap`."""
     dims = plan.shape.dims
     for _ in range(steps):
-        plan.laplace(x, scale=float(n * n), out=lap, stream=stream)
-        plan.forward(lap, out=y, scratch=scratch, stream=stream)
+        if fused:
+            plan.laplace_forward(x, float(n * n), out=y, scratch=scratch, stream=stream)
+        else:
+            plan.laplace(x, scale=float(n * n), out=lap, stream=stream)
+            plan.forward(lap, out=y, scratch=scratch, stream=stream)
         plan.inverse(y, 1.0 / float(n) ** dims, out=z, scratch=scratch, stream=stream)
 
 
-def time_path(plan, x, n, steps, warmup, torch, label):
+def time_path(plan, x, n, steps, warmup, torch, label, fused=True):
     """Time exactly `steps` steps with no per-launch instrumentation (the step
     time and `value`), then the same `steps` again with the plan's per-launch
     HIP events, one recorded on the launch stream after every kernel (the
-    stage breakdown and the dominant kernel's launch durations)."""
+    stage breakdown and the dominant kernel's launch durations).  Returns the
+    step's outputs (Laplacian, forward, round trip); the fused step does not
+    write its Laplacian, which is computed after the timed steps."""
     lap = torch.empty_like(x)
     y = torch.empty_like(x)
     z = torch.empty_like(x)
     scratch = torch.empty(plan.scratch_bytes // x.element_size(), dtype=x.dtype, device="cuda")
     stream = torch.cuda.current_stream()
-    run_steps(plan, x, lap, y, z, scratch, n, warmup, torch, stream)
+    run_steps(plan, x, lap, y, z, scratch, n, warmup, torch, stream, fused)
     torch.cuda.synchronize()
 
     def timed(instrumented):
@@ -83,7 +98,7 @@ def time_path(plan, x, n, steps, warmup, torch, label):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         start.record(stream)
-        run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream)
+        run_steps(plan, x, lap, y, z, scratch, n, steps, torch, stream, fused)
         stop.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -95,13 +110,17 @@ def time_path(plan, x, n, steps, warmup, torch, label):
     ms_instr, _ = timed(True)
     stages = plan.timing_read()
     plan.timing_reset()
-    log(f"{label}: {ms:.3f} ms/step (wall {1e3 * wall / steps:.3f} ms/step; with per-launch events "
-        f"{ms_instr:.3f}); stages " + ", ".join(f"{k}: {c} x {t / c:.3f} ms" for k, (c, t) in stages.items()))
+    if fused:
+        plan.laplace(x, scale=float(n * n), out=lap, stream=stream)
+        torch.cuda.synchronize()
+    log(f"{label}{' (fused Laplacian)' if fused else ''}: {ms:.3f} ms/step (wall {1e3 * wall / steps:.3f} ms/step; "
+        f"with per-launch events {ms_instr:.3f}); stages "
+        + ", ".join(f"{k}: {c} x {t / c:.3f} ms" for k, (c, t) in stages.items()))
     time_path.last_instrumented = ms_instr
     return ms, stages, (lap, y, z)
 
 
-def time_graph(plan, x, n, steps, torch):
+def time_graph(plan, x, n, steps, torch, fused=True):
     """One step captured in a HIP graph (torch.cuda.graph on the plan's
     stream-ordered launches), replayed `steps` times."""
     lap, y, z = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
@@ -109,12 +128,12 @@ def time_graph(plan, x, n, steps, torch):
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        run_steps(plan, x, lap, y, z, scratch, n, 2, torch, None)
+        run_steps(plan, x, lap, y, z, scratch, n, 2, torch, None, fused)
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
-        run_steps(plan, x, lap, y, z, scratch, n, 1, torch, None)
+        run_steps(plan, x, lap, y, z, scratch, n, 1, torch, None, fused)
     for _ in range(3):
         g.replay()
     torch.cuda.synchronize()
@@ -336,6 +355,7 @@ def load_traffic(name):
 
 KERNELS = {"line": "line_fast_kernel (fused FFT1a + exchange + twiddle + FFT1b direction pass, forward and inverse "
                    "instantiations)",
+           "line_laplace": "line_fast_kernel with the 5-point Laplacian formed ahead of the forward x pass",
            "repack": "repack_tiled_kernel", "laplace": "laplace_strip_kernel",
            "stage_a": "stage_a_kernel", "stage_b": "stage_b_kernel"}
 
@@ -383,8 +403,10 @@ def leg_3d(args, torch, Plan):
     x32 = x64.to(torch.complex64)
     del x64
     x64r = x32.to(torch.complex128)
-    ms32, st32, outs32 = time_path(plan32, x32, n, steps, 2, torch, "3D fp32")
-    ms64, st64, outs64 = time_path(plan64, x64r, n, steps, 2, torch, "3D fp64")
+    # the reference's three calls (3D has no fused Laplacian pass: it would
+    # need the planes before and after as well)
+    ms32, st32, outs32 = time_path(plan32, x32, n, steps, 2, torch, "3D fp32", False)
+    ms64, st64, outs64 = time_path(plan64, x64r, n, steps, 2, torch, "3D fp64", False)
     errs = compare_outputs(plan32, plan64, x32, x64r, outs32, outs64, n, torch)
     del outs32, outs64
     der = time_derivatives(plan32, x32, n, 10, torch)
@@ -393,6 +415,7 @@ def leg_3d(args, torch, Plan):
     res = {
         "workload": f"3D N={n} P={p} B={b} (8 virtual ranks): 7-point Laplacian -> forward FFT -> inverse FFT",
         "field": desc,
+        "step_chain": "zm_laplace, zm_forward, zm_inverse (9 kernel sweeps)",
         "fp32_ms_per_step": round(ms32, 4), "fp32_gflops": round(flops / (ms32 * 1e-3) / 1e9, 1),
         "fp64_ms_per_step": round(ms64, 4), "fp64_gflops": round(flops / (ms64 * 1e-3) / 1e9, 1),
         "speedup_fp32_vs_fp64_gpu": round(ms64 / ms32, 3),
@@ -416,6 +439,9 @@ def main():
     ap.add_argument("--p", type=int, default=None, help="default 4 (2D) / 2 (3D)")
     ap.add_argument("--b", type=int, default=None, help="default 32 (2D) / 16 (3D)")
     ap.add_argument("--two-stage", action="store_true", help="force the two-kernel-per-direction path")
+    ap.add_argument("--separate", action="store_true",
+                    help="time the reference's three calls per step (Laplacian, forward, inverse) as the headline "
+                         "instead of the chain with the Laplacian fused into the forward transform")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison paths")
     ap.add_argument("--no-mfma", action="store_true", help="skip the split-precision MFMA leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D configuration leg")
@@ -481,7 +507,8 @@ def main():
     del x64
     torch.cuda.synchronize()
 
-    ms, stages, outs = time_path(plan, x, n, args.steps, args.warmup, torch, args.precision)
+    fused = not (args.separate or args.two_stage)
+    ms, stages, outs = time_path(plan, x, n, args.steps, args.warmup, torch, args.precision, fused)
     esize = x.element_size()
     field = n ** dims
     flops = 2 * 5.0 * field * math.log2(field)
@@ -521,7 +548,41 @@ def main():
 
     extra = {}
     if args.graph:
-        extra["graph_ms_per_step"] = round(time_graph(plan, x, n, args.steps, torch), 4)
+        extra["graph_ms_per_step"] = round(time_graph(plan, x, n, args.steps, torch, fused), 4)
+    # the step's chain: zm_laplace_forward (on 2D FP32 plans with N >= 4096
+    # the Laplacian is formed inside the forward transform's x pass) then
+    # zm_inverse, or the reference's three calls; the other one is timed
+    # beside it.  A kernel sweep is one launch that reads and writes the field.
+    def sweeps_of(st):
+        return sum(c for c, _ in st.values()) // args.steps
+    extra["step_chain"] = {
+        "fused": fused,
+        "kernel_sweeps_per_step": sweeps_of(stages),
+        "description": ("zm_laplace_forward then zm_inverse" if fused
+                        else "zm_laplace, zm_forward, zm_inverse (zmodel/main.cu:50-78)"),
+    }
+    if not args.two_stage:
+        ksteps = max(3, min(args.steps, 10))
+        msx, stx, outsx = time_path(plan, x, n, ksteps, 2, torch, args.precision, not fused)
+        del outsx
+        extra["other_chain"] = {
+            "fused": not fused, "ms_per_step": round(msx, 4), "gflops": round(flops / (msx * 1e-3) / 1e9, 1),
+            "kernel_sweeps_per_step": sum(c for c, _ in stx.values()) // ksteps,
+            "stages": stage_roofline(stx, field * esize),
+        }
+    if "line_laplace" in stages:
+        # the fused pass does the work of two sweeps (the Laplacian's and the
+        # x pass's) in one read and one write of the field
+        c, t = stages["line_laplace"]
+        avg = t / c
+        extra["fused_laplace_pass"] = {
+            "kernel": KERNELS["line_laplace"],
+            "avg_launch_ms": round(avg, 4),
+            "achieved_gbs": round(per_launch / (avg * 1e-3) / 1e9, 1),
+            "replaced_sweeps_gbs": round(2 * per_launch / (avg * 1e-3) / 1e9, 1),
+            "basis": "achieved: one read and one write of the field; replaced_sweeps: the Laplacian's and the x "
+                     "pass's reads and writes, which the fused pass does in one",
+        }
     extra.update({
         "ms_per_step_with_launch_events": round(time_path.last_instrumented, 4),
         "stage_ms": {k: round(t / c, 4) for k, (c, t) in stages.items()},
@@ -538,10 +599,16 @@ def main():
     outs64 = None
     if not args.no_fp64 and args.precision != "fp64":
         x64r = x.to(torch.complex128)
-        ms64, st64, outs64 = time_path(plan64, x64r, n, max(3, min(args.steps, 10)), 2, torch, "fp64")
+        # the FP64 oracle: the plain FP64 GPU path with the reference's three
+        # calls (its Laplacian bit for bit the reference's); every FP32 output
+        # is compared with it
+        ms64s, st64s, outs64 = time_path(plan64, x64r, n, max(3, min(args.steps, 10)), 2, torch, "fp64", False)
         errs = compare_outputs(plan, plan64, x, x64r, outs, outs64, n, torch)
         errs["round_trip_vs_fp64_laplace"] = rel_errors(outs[2], outs64[0], torch)
         extra["errors_vs_fp64_gpu"] = errs
+        # FP64 has no fused Laplacian pass (its register sets do not fit), so
+        # its three calls are also its fastest step
+        ms64, st64 = ms64s, st64s
         extra["fp64_ms_per_step"] = round(ms64, 4)
         extra["fp64_stages"] = stage_roofline(st64, field * 16)
         extra["speedup_vs_fp64_gpu"] = round(ms64 / ms, 3)
@@ -555,10 +622,10 @@ def main():
         # the post-exchange buffer (zmodel/transforms/fft.cu:199-339)
         if not args.two_stage:
             plan64s = Plan(dims, n, p, b, precision="fp64", two_stage=True)
-            ms64s, _, outs64s = time_path(plan64s, x64r, n, max(3, min(args.steps, 10)), 2, torch,
-                                          "fp64 two-stage")
-            extra["fp64_two_stage_ms_per_step"] = round(ms64s, 4)
-            extra["speedup_vs_fp64_two_stage_gpu"] = round(ms64s / ms, 3)
+            ms64t, _, outs64s = time_path(plan64s, x64r, n, max(3, min(args.steps, 10)), 2, torch,
+                                          "fp64 two-stage", False)
+            extra["fp64_two_stage_ms_per_step"] = round(ms64t, 4)
+            extra["speedup_vs_fp64_two_stage_gpu"] = round(ms64t / ms, 3)
             del outs64s, plan64s
         # the split-precision MFMA variant of the same step
         if not args.no_mfma and args.precision == "fp32" and not args.two_stage:
@@ -568,7 +635,7 @@ def main():
                 log(f"no fp32_mfma plan for this shape: {e}")
                 pm = None
             if pm is not None:
-                msm, stm, outsm = time_path(pm, x, n, args.steps, args.warmup, torch, "fp32_mfma")
+                msm, stm, outsm = time_path(pm, x, n, args.steps, args.warmup, torch, "fp32_mfma", fused)
                 errm = compare_outputs(pm, plan64, x, x64r, outsm, outs64, n, torch)
                 del outsm
                 c, t = stm["line"]
@@ -593,7 +660,7 @@ def main():
     if not args.no_cpu_baseline:
         torch.cuda.empty_cache()
         if outs64 is None:
-            _, _, outs64 = time_path(plan64, x.to(torch.complex128), n, 1, 1, torch, "fp64")
+            _, _, outs64 = time_path(plan64, x.to(torch.complex128), n, 1, 1, torch, "fp64", False)
         # the CPU legs run on the input of the FP64 GPU path: the field as
         # rounded for the timed precision, promoted back to complex128
         nat_in = plan64.packed_to_natural(x.to(torch.complex128)).cpu().numpy()
@@ -623,7 +690,8 @@ def main():
                          f"Laplacian (h=1/N) -> forward FFT -> inverse FFT (1/N^{dims}), complex "
                          f"{'128' if args.precision == 'fp64' else '64'}"),
             "n": n, "p": p, "b": b, "precision": args.precision,
-            "path": "two-stage" if args.two_stage else "fused",
+            "path": "two-stage" if args.two_stage else ("Laplacian fused into the forward x pass" if fused
+                                                        else "three calls"),
             "field": field_desc,
         },
         "roofline": roofline,

@@ -134,15 +134,11 @@ zm_status zm_forward(zm_plan* plan, const void* in, void* out, void* scratch, vo
 /* Forward transform of `scale` times the periodic 5-point (2D) / 7-point (3D)
  * Laplacian of `in`: the reference's laplace followed by forward_fft
  * (zmodel/main.cu:50-63, laplace.cu:73-169 / laplace3d.cu:90-225, fft.cu:
- * 199-339), in one chain of three (2D) or five (3D) passes instead of four or
- * six.  The directions before the last transform `in` itself; the last
- * direction's line pass forms the Laplacian of what it reads, multiplying by
- * the exact symbol -4 sin^2(pi k / N) along the transformed directions and
- * taking the second difference along its own, then transforms it.  Results
- * equal zm_forward(zm_laplace(in)) up to rounding (not bit for bit).  Plans
- * without ZM_PATH_FAST_LINE run the Laplacian and then the forward chain
- * (four / six passes); ZM_PATH_TWO_STAGE plans get ZM_ERR_UNSUPPORTED.
- * Buffers as zm_forward. */
+ * 199-339), bit for bit zm_forward(zm_laplace(in)) with `scale`.  On 2D FP32
+ * plans with the specialised path and N >= 4096 the Laplacian is formed inside
+ * the x pass (three sweeps of the field instead of four); elsewhere it is the
+ * two calls.  ZM_PATH_TWO_STAGE plans get ZM_ERR_UNSUPPORTED.  Buffers as
+ * zm_forward. */
 zm_status zm_laplace_forward(zm_plan* plan, const void* in, void* out, double scale, void* scratch, void* stream);
 
 /* Scratch bytes available to zm_forward_noalloc on each GPU. */
@@ -306,7 +302,8 @@ typedef enum {
     ZM_STAGE_DERIV = 5,    /* dx4 / dy4 */
     ZM_STAGE_SPECTRAL = 6,
     ZM_STAGE_CONVERT = 7,  /* layout conversion and index tables */
-    ZM_STAGE_COUNT = 8
+    ZM_STAGE_LINE_LAPLACE = 8, /* zm_laplace_forward's x pass with the Laplacian formed in it */
+    ZM_STAGE_COUNT = 9
 } zm_stage;
 
 /* enable != 0 starts recording one event after every subsequent kernel launch

@@ -34,7 +34,7 @@ ZM_FLAG_CALLER_SCRATCH = 2
 ZM_PATH_TWO_STAGE, ZM_PATH_FAST_LINE, ZM_PATH_MFMA = 1, 2, 4
 ZM_COMPLEX, ZM_REAL = 0, 1
 ZM_SPECTRAL_POISSON, ZM_SPECTRAL_HEAT, ZM_SPECTRAL_SCALE, ZM_SPECTRAL_STENCIL, ZM_SPECTRAL_POISSON_STENCIL = range(5)
-STAGES = ["line", "stage_a", "stage_b", "repack", "laplace", "deriv", "spectral", "convert"]
+STAGES = ["line", "stage_a", "stage_b", "repack", "laplace", "deriv", "spectral", "convert", "line_laplace"]
 
 
 class ZmShape(ctypes.Structure):

@@ -51,16 +51,16 @@ hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const Repack
 template <typename T>
 bool fast_line_supported(int vec, int m);
 template <typename T>
-int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapy);
+int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapx);
+// whether the fused Laplacian + x pass (lapx) exists for this (VEC, M) pair
+template <typename T>
+bool fast_line_lapx(int vec, int m, bool mf);
 // mf: radix-16 passes as split-precision FP16 MFMA GEMMs (FP32 only).
-// lap_sigma (forward only; null: none): the pass transforms `scale` times the
-// Laplacian of its input, the earlier directions being already transformed
-// (the fused Laplacian + forward transform); lap_sigma[l] is line l's symbol
-// of the transformed directions (laplace_sigma_table).
+// lapx (2D forward x pass, fast_line_lapx): transform `scale` times the
+// 5-point Laplacian of the input (the fused Laplacian + forward transform).
 template <typename T>
 hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
-                            int batch, bool fwd, double scale, int grid_x, bool mf, const T* lap_sigma,
-                            hipStream_t stream);
+                            int batch, bool fwd, double scale, int grid_x, bool mf, bool lapx, hipStream_t stream);
 int line_table_entries(int vec, int m);
 hipError_t launch_line_tables(double2* t64, float2* t32, int vec, int m, hipStream_t stream);
 

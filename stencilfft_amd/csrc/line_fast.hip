@@ -85,58 +85,6 @@ __device__ __forceinline__ int pre_exp(int u, int r) {
     return ((j + r * nbf) * kk) & (N - 1);
 }
 
-// The Laplacian folded into the forward transform's last direction pass.
-// By then the other directions are transformed: along them the periodic
-// 3-point second difference u(+1) + u(-1) - 2 u is the multiplication of each
-// mode by sigma(k) = -4 sin^2(pi k / N) (its exact symbol), so the 5-point
-// (7-point) Laplacian of the input is, for each line of the last direction,
-// sigma_line u + (u(g + 1) + u(g - 1) - 2 u(g)) along the line, with
-// sigma_line the sum over the transformed directions.  Element v of vector m
-// of a line (LDS index m VEC + v) is the line coordinate g = v M + m (the
-// packed layout of zmodel/utils.h:99-128, member rank m / B, vector m mod B,
-// lane v), so its neighbours are vectors m -+ 1, and at m = 0 / M - 1 the
-// neighbouring lane's other end (periodic in g).
-template <typename T>
-struct LineStencil {
-    const T* sig;  // sigma of each line of the group (LDS)
-    T scale;       // the Laplacian's scale (1 / h^2)
-};
-
-template <typename T>
-__device__ __forceinline__ Cx<T> lap_combine_line(Cx<T> a, Cx<T> b, Cx<T> c, T sig, T scale) {
-    const T re = ((a.x - c.x) + (b.x - c.x)) + sig * c.x;
-    const T im = ((a.y - c.y) + (b.y - c.y)) + sig * c.y;
-    return Cx<T>{re * scale, im * scale};
-}
-
-// Element p (LDS index, unpadded) of vector m with 0 < m < M - 1: its line
-// neighbours are p -+ VEC, at a constant distance in the padded index when VEC
-// is a multiple of the padding period.
-template <typename T, int LVEC, int LN>
-__device__ __forceinline__ Cx<T> line_laplacian_inner(const Cx<T>* src, int p, Cx<T> c, T sig, T scale) {
-    constexpr int VEC = 1 << LVEC, PS = kPadShift<T, LN>;
-    Cx<T> a, b;
-    if constexpr ((VEC & ((1 << PS) - 1)) == 0) {
-        constexpr int PV = VEC + (VEC >> PS);
-        const int pp = fidx<T, LN>(p);
-        a = src[pp - PV];
-        b = src[pp + PV];
-    } else {
-        a = src[fidx<T, LN>(p - VEC)];
-        b = src[fidx<T, LN>(p + VEC)];
-    }
-    return lap_combine_line<T>(a, b, c, sig, scale);
-}
-
-template <typename T, int LVEC, int LM, int LN>
-__device__ __forceinline__ Cx<T> line_laplacian(const Cx<T>* src, int line, int m, int v, Cx<T> c, T sig, T scale) {
-    constexpr int VEC = 1 << LVEC, M = 1 << LM;
-    const int lb = line << LN;
-    const int ia = m > 0 ? lb + ((m - 1) << LVEC) + v : lb + ((M - 1) << LVEC) + ((v - 1) & (VEC - 1));
-    const int ib = m < M - 1 ? lb + ((m + 1) << LVEC) + v : lb + ((v + 1) & (VEC - 1));
-    return lap_combine_line<T>(src[fidx<T, LN>(ia)], src[fidx<T, LN>(ib)], c, sig, scale);
-}
-
 // One Stockham radix-R pass over the line-group held in LDS.
 //   CONTIG: transforms are contiguous runs of length n (the VEC-point DFTs,
 //           batch index bt = (line, r)); lanes run along j.
@@ -152,12 +100,10 @@ __device__ __forceinline__ Cx<T> line_laplacian(const Cx<T>* src, int line, int 
 // whole W_N (FULLTAB) or the two-level A/B pair.  `ts` is this stage's
 // per-pass table, W_{NS R}^(jm r) at r*NS + jm (jm runs along the lanes, so
 // both reads are free of bank conflicts).
-// STN: replace each element by the line Laplacian (line_laplacian) as it is
-// read; first pass of the VEC-point DFTs only.
 template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool TOGLOBAL, bool FULLTAB, bool OOP, bool PTW, bool STN = false, typename Sink>
+          bool TOGLOBAL, bool FULLTAB, bool OOP, bool PTW, typename Sink>
 __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, const Cx<T>* ta,
-                                          const Cx<T>* tb, int tid, Sink&& sink, LineStencil<T> stn = {}) {
+                                          const Cx<T>* tb, int tid, Sink&& sink) {
     using C = Cx<T>;
     constexpr int n = 1 << LN_T;
     constexpr int nbf = n / R;
@@ -174,10 +120,10 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
 #pragma unroll
     for (int k = 0; k < BPT; k++) {
         const int u = tid + k * NT;
-        int base, j, kk, bt = 0;
+        int base, j, kk;
         if (CONTIG) {
             j = u & (nbf - 1);
-            bt = u >> LNBF;  // (line, r) when n == VEC
+            const int bt = u >> LNBF;  // (line, r) when n == VEC
             base = bt << LN_T;
             kk = bt & ((N >> LN_T) - 1);  // r for the VEC transforms
         } else {
@@ -190,21 +136,6 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
         }
 #pragma unroll
         for (int r = 0; r < R; r++) v[k][r] = src[fidx<T, LN>(base + (j + r * nbf) * ES)];
-        if constexpr (STN) {
-            static_assert(CONTIG && LN_T == LVEC && NS == 1 && !PRE, "the line Laplacian precedes the VEC DFTs");
-            constexpr int LM_ = LN - LVEC;
-            const int m = bt & ((1 << LM_) - 1), line = bt >> LM_;
-            const T sg = stn.sig[line];
-            if (m > 0 && m < (1 << LM_) - 1) {
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    v[k][r] = line_laplacian_inner<T, LVEC, LN>(src, base + j + r * nbf, v[k][r], sg, stn.scale);
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    v[k][r] = line_laplacian<T, LVEC, LM_, LN>(src, line, m, j + r * nbf, v[k][r], sg, stn.scale);
-            }
-        }
         if (PRE) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -274,8 +205,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
 // stages it into LDS (largest |re|, |im| of the group in [2^6, 2^7),
 // MFMA_SCALE_LOG2) and undoes it exactly on the final store.  A radix-R pass
 // grows magnitudes by at most R, and at most 16 * 8 of growth precedes the
-// last MFMA pass of a direction (the fused Laplacian's growth of at most 12
-// is offset by a group scale 2^4 lower), so every hi operand stays below 2^14 and
+// last MFMA pass of a direction, so every hi operand stays below 2^14 and
 // every lo operand below 2^12 * 2^(14 - 11) = 2^15 < 65504; values 2^-30
 // below the group's largest still keep their absolute error under 2^-31 of it.
 // ---------------------------------------------------------------------------
@@ -325,10 +255,9 @@ __device__ __forceinline__ Mfma16W mfma16_weights(int lane) {
 }
 
 template <typename T, int LN_T, int NS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool TOGLOBAL, bool FULLTAB, bool OOP, bool STN = false, typename Sink>
+          bool TOGLOBAL, bool FULLTAB, bool OOP, typename Sink>
 __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, const Cx<T>* ta,
-                                            const Cx<T>* tb, int tid, const Mfma16W* wtab, Sink&& sink,
-                                            LineStencil<T> stn = {}) {
+                                            const Cx<T>* tb, int tid, const Mfma16W* wtab, Sink&& sink) {
     static_assert(sizeof(T) == 4, "split-precision MFMA pass is FP32 only");
     static_assert(OOP || TOGLOBAL, "the MFMA pass writes each group as soon as it is done");
     using C = Cx<T>;
@@ -350,10 +279,10 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
 #pragma unroll 1
     for (int q = 0; q < 4; q++) {
         const int u = wave * 64 + q * 16 + col;  // transform handled by this column
-        int base, j, kk, bt = 0;
+        int base, j, kk;
         if (CONTIG) {
             j = u & (nbf - 1);
-            bt = u >> LNBF;
+            const int bt = u >> LNBF;
             base = bt << LN_T;
             kk = bt & ((N >> LN_T) - 1);
         } else {
@@ -369,16 +298,6 @@ __device__ __forceinline__ void mfma_pass16(const Cx<T>* src, Cx<T>* dst, const 
         for (int i = 0; i < 4; i++) {
             const int r = 4 * g + i;
             x[i] = src[fidx<T, LN>(base + (j + r * nbf) * ES)];
-            if constexpr (STN) {
-                static_assert(CONTIG && LN_T == LVEC && !PRE, "the line Laplacian precedes the VEC DFTs");
-                constexpr int LM_ = LN - LVEC;
-                const int m = bt & ((1 << LM_) - 1), line = bt >> LM_;
-                if (m > 0 && m < (1 << LM_) - 1)
-                    x[i] = line_laplacian_inner<T, LVEC, LN>(src, base + j + r * nbf, x[i], stn.sig[line], stn.scale);
-                else
-                    x[i] = line_laplacian<T, LVEC, LM_, LN>(src, line, m, j + r * nbf, x[i], stn.sig[line],
-                                                            stn.scale);
-            }
             if (PRE) {
                 const int e = ((j + r * nbf) * kk) & (N - 1);
                 if constexpr (FULLTAB) x[i] = ctw<FWD>(x[i], ta[e]);
@@ -435,22 +354,20 @@ struct NoSink {
 // Run passes PASS..NPASS-1 of a length-2^LN_T transform, all in LDS.  With
 // OOP the passes alternate between the buffers `cur` and `alt`; the return
 // value is the buffer that holds the result.
-// STN: the line Laplacian on the first pass (see line_laplacian).
 template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool FULLTAB, bool MF, bool OOP, int HOFF = 0, bool STN = false, typename Hook = NoHook>
+          bool FULLTAB, bool MF, bool OOP, int HOFF = 0, typename Hook = NoHook>
 __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const Cx<T>* ts, const Cx<T>* ta,
-                                                  const Cx<T>* tb, int tid, const Mfma16W* w, Hook&& hook = Hook{},
-                                                  LineStencil<T> stn = {}) {
+                                                  const Cx<T>* tb, int tid, const Mfma16W* w, Hook&& hook = Hook{}) {
     if constexpr (PASS < NPASS) {
         constexpr int R = fast::Radix<LN_T, PASS>::value;
         constexpr int NS = 1 << (4 * PASS);
         Cx<T>* dst = OOP ? alt : cur;
         if constexpr (MF && R == 16)
-            mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP,
-                        STN && PASS == 0>(cur, dst, ts, ta, tb, tid, w, NoSink{}, stn);
+            mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
+                cur, dst, ts, ta, tb, tid, w, NoSink{});
         else
             fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP,
-                      kPerThreadPre<MF, FULLTAB, LN, LE>, STN && PASS == 0>(cur, dst, ts, ta, tb, tid, NoSink{}, stn);
+                      kPerThreadPre<MF, FULLTAB, LN, LE>>(cur, dst, ts, ta, tb, tid, NoSink{});
         hook(std::integral_constant<int, HOFF + PASS + 1>{});
         return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP, HOFF>(
             dst, OOP ? cur : alt, ts, ta, tb, tid, w, hook);
@@ -472,15 +389,49 @@ __device__ __forceinline__ void last_pass(const Cx<T>* cur, const Cx<T>* ts, con
                   kPerThreadPre<MF, FULLTAB, LN, LE>>(cur, nullptr, ts, ta, tb, tid, sink);
 }
 
+// The periodic 5-point Laplacian of global row gr at element e of the line,
+// from the rows above / below (this thread's elements of rows gr -+ 1) and
+// the row itself in LDS: its x neighbours are the line's vectors m -+ 1 (line
+// coordinate x = v M + m, v the lane; at m = 0 / M - 1 the neighbouring lane's
+// other end, the reference's lane shift at the global x edge, laplace.cu:148,
+// 160).  FP64 sums in the reference's order (laplace.cu:117-167), FP32 in the
+// first-difference order of the stencil kernels, both then times `scale`: the
+// same operations as zm_laplace, so the values are bit for bit its own.
+template <typename T, int LVEC, int LM, int LN>
+__device__ __forceinline__ Cx<T> row_laplacian(const Cx<T>* row, int e, Cx<T> a, Cx<T> b, Cx<T> c, T scale,
+                                               bool apply_scale) {
+#pragma clang fp contract(off)
+    using C = Cx<T>;
+    constexpr int VEC = 1 << LVEC, M = 1 << LM;
+    const int m = e >> LVEC, v = e & (VEC - 1);
+    const int il = m > 0 ? e - VEC : ((M - 1) << LVEC) + ((v - 1) & (VEC - 1));
+    const int ir = m < M - 1 ? e + VEC : ((v + 1) & (VEC - 1));
+    const C l = row[fidx<T, LN>(il)], r = row[fidx<T, LN>(ir)];
+    C s;
+    if constexpr (sizeof(T) == 8) {
+        s = C{(T)0, (T)0};
+        s = C{s.x + a.x, s.y + a.y};
+        s = C{s.x + b.x, s.y + b.y};
+        s = C{s.x + l.x, s.y + l.y};
+        s = C{s.x + r.x, s.y + r.y};
+        s = C{s.x - c.x * (T)4, s.y - c.y * (T)4};
+    } else {
+        const T yx = (a.x - c.x) + (b.x - c.x), yy = (a.y - c.y) + (b.y - c.y);
+        const T xx = (l.x - c.x) + (r.x - c.x), xy = (l.y - c.y) + (r.y - c.y);
+        s = C{yx + xx, yy + xy};
+    }
+    if (apply_scale) s = C{s.x * scale, s.y * scale};
+    return s;
+}
+
 // Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
 // iteration.  a.lpb is the number of iterations (line groups) per workgroup.
-// LAPY (forward, last direction only): transform scale * Laplacian of the
-// data the earlier directions transformed (line_laplacian), `scale` being the
-// Laplacian's scale.
-template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPY = false>
+// LAPX (2D forward x pass, one-line groups): transform scale * the 5-point
+// Laplacian of the input instead of the input (zm_laplace_forward).
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPX = false>
 __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     line_fast_kernel(const Cx<T>* in, Cx<T>* out, Geo g, LineArgs a, const Cx<T>* __restrict__ tables, T scale,
-                     int apply_scale, int groups, const T* __restrict__ lap_sigma) {
+                     int apply_scale, int groups) {
     using C = Cx<T>;
     constexpr int VEC = 1 << LVEC, M = 1 << LM;
     constexpr int LN = LVEC + LM, N = 1 << LN;
@@ -540,8 +491,6 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     // the tables in LDS
     Mfma16W* w = reinterpret_cast<Mfma16W*>(tabs + TP + (FULLTAB ? N : NTWO - S));
     float* wmax = reinterpret_cast<float*>(w + 64);
-    // LAPY: sigma of each line of the group, after the maxima slots
-    T* sigt = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(wmax) + MFMA_MAX_BYTES);
     constexpr int NW = NT / 64;
     static_assert(!MF || NW <= 16, "maxima slots");
     if constexpr (MF) {
@@ -626,91 +575,23 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             else lds[fidx<T, LN>(li)] = pre[k];
         }
     };
-    // LAPY: sigma of the group's lines, from the plan's per-line table
-    auto stage_sigma = [&](int grp) {
-        if constexpr (LAPY) {
-            for (int l = tid; l < (1 << LLPI); l += NT) sigt[l] = lap_sigma[((int64_t)grp << LLPI) + l];
-        }
-    };
 
-    int grp = blockIdx.x;
-    int it = 0;
-    if (grp < groups) {
-        fetch(grp);
-        if constexpr (MF) group_max(0);
-    }
-    __syncthreads();  // tables, maxima
-    for (; grp < groups; grp += gridDim.x, ++it) {
-        // The compiler hoists every per-thread global and LDS address out of
-        // this loop and keeps them live.  An opaque tid per iteration makes
-        // them recomputed per group instead, where that was measured faster:
-        //  * FP64, N = 8192: the hoisted addresses spill 6 to 87 VGPRs at the
-        //    256-VGPR limit; recomputed, 154-163 VGPRs and no spills (the
-        //    pass 0.562 -> 0.467 ms);
-        //  * FP32 vector ALU, several lines per group: 228 -> 102-120 VGPRs,
-        //    four 256-thread workgroups per CU instead of two (3D N = 512:
-        //    0.413 -> 0.401 ms).
-        // Elsewhere it only costs (FP64 3D N = 512: 0.725 -> 0.835 ms), except
-        // in the fused Laplacian passes, whose neighbour reads spill with the
-        // hoisted addresses live.
-        constexpr bool kRecompute = (sizeof(T) == 8 && LN >= 13) || (sizeof(T) == 4 && !MF && LLPI > 0) || LAPY;
-        if constexpr (kRecompute) asm volatile("" : "+v"(tid));
-        if constexpr (MF) {
-            float m = 0.0f;
-#pragma unroll
-            for (int v = 0; v < NW; v++) m = fmaxf(m, wmax[(it & 1) * 16 + v]);
-            int ex;
-            (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
-            // a group of FP32 denormals or tiny normals would need a scale
-            // beyond 2^127: clamp, so s0 stays finite (its largest value
-            // then lands below 2^7, which costs no range)
-            // LAPY: the line Laplacian grows magnitudes by up to 12 (|sigma| <= 8
-            // plus the second difference's 4) before the first MFMA pass, so
-            // the group sits 2^4 lower; its scale is undone with the group's
-            // on the final store
-            constexpr int SCL = MFMA_SCALE_LOG2 - (LAPY ? 4 : 0);
-            ex = ex < SCL - 127 ? SCL - 127 : ex;
-            s0 = ldexpf(1.0f, SCL - ex);
-            inv_s0 = ldexpf(1.0f, ex - SCL);
-            if constexpr (LAPY) inv_s0 *= scale;
-        }
-        stage();
-        stage_sigma(grp);
-        __syncthreads();
-        const int nxt = grp + gridDim.x;
-        // The next group's loads are spread over the pass boundaries (HN
-        // points: after staging and after every LDS pass before the last); a
-        // burst of EPT loads per thread stalls instruction issue.  Measured:
-        //  * FP32 vector ALU: always (N = 8192: 0.234 -> 0.213 ms);
-        //  * one-line groups (N >= 4096) of FP64 (N = 8192: 0.466 -> 0.401
-        //    ms) and MFMA (0.289 -> 0.260 ms);
-        //  * not the multi-line groups of FP64 or MFMA, which were slower
-        //    spread (3D N = 512: FP64 0.725 -> 0.871 ms, MFMA 0.456 -> 0.470).
-        // MF's maxima of the next group are taken after the last pass, when
-        // all of its loads were issued.
-        constexpr bool SPREAD = (sizeof(T) == 4 && !MF) || LLPI == 0;
-        constexpr int NP1 = FWD ? NPV : NPM;
-        constexpr int NP2 = STAGED ? (FWD ? NPM : NPV) : ((FWD ? NPM : NPV) - 1);
-        constexpr int HN = 1 + NP1 + NP2;
-        auto hook = [&](auto hc) {
-            constexpr int h = decltype(hc)::value;
-            if constexpr (SPREAD && h < HN) {
-                constexpr int K0 = h * EPT / HN, K1 = (h + 1) * EPT / HN;
-                if constexpr (K1 > K0) {
-                    if (nxt < groups)
-                        fetch_part(std::integral_constant<int, K0>{}, std::integral_constant<int, K1>{}, nxt);
-                }
-            }
-        };
-        if constexpr (SPREAD) hook(std::integral_constant<int, 0>{});
-        else if (nxt < groups) fetch(nxt);
+    // Spread prefetch: HN hook points (after staging and after every LDS pass
+    // before the last) take a slice each of the next group's loads.
+    constexpr bool SPREAD = (sizeof(T) == 4 && !MF) || LLPI == 0;
+    constexpr int NP1 = FWD ? NPV : NPM;
+    constexpr int NP2 = STAGED ? (FWD ? NPM : NPV) : ((FWD ? NPM : NPV) - 1);
+    constexpr int HN = 1 + NP1 + NP2;
 
+    // The line group's transform, from the staged LDS data to HBM, with the
+    // next group's loads spread over its pass boundaries by `hook`.
+    auto transform_group = [&](int grp, auto& hook) __attribute__((always_inline)) {
         C* const gd = dst0 + group_base(grp);
 
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
-            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0, LAPY>(
-                lds, lds2, tsv, ta, tb, tid, w, hook, LineStencil<T>{sigt, MF ? (T)1 : scale});
+            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0>(
+                lds, lds2, tsv, ta, tb, tid, w, hook);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
             if constexpr (STAGED) {
@@ -783,6 +664,145 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                     });
             }
         }
+    };
+
+    if constexpr (LAPX) {
+        // zm_laplace_forward's x pass: the workgroup walks a contiguous chunk
+        // of global rows and transforms each row's Laplacian.  The rows above
+        // and below come from registers (this thread's elements of the
+        // previous row and of the prefetched next row), the x neighbours from
+        // the row staged in LDS, so the Laplacian costs no HBM traffic beyond
+        // the two rows bordering the chunk.  Rows run two ahead: while row gr
+        // is transformed, row gr + 2's loads are in flight into the register
+        // set row gr came from.
+        static_assert(FWD && LLPI == 0 && !MF, "the fused Laplacian runs ahead of one-line forward x passes");
+        C rowa[EPT], rowb[EPT], up[EPT];
+        const int chunk = (groups + (int)gridDim.x - 1) / (int)gridDim.x;
+        const int r0 = (int)blockIdx.x * chunk;
+        const int r1 = r0 + chunk < groups ? r0 + chunk : groups;
+        // global row gr -> its line: exchange group o = rid, local row lr
+        // (zmodel/utils.h:176-185)
+        auto line_of = [&](int gr) -> int {
+            gr &= g.n - 1;
+            const int rid = (gr >> g.lb) & (g.p - 1);
+            const int lr = ((gr >> (g.lb + lp)) << g.lb) + (gr & (g.b - 1));
+            return (rid << ll) + lr;
+        };
+        auto fetch_row = [&](C* dst, int gr, auto k0c, auto k1c) __attribute__((always_inline)) {
+            constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
+            const C* gb = src0 + group_base(line_of(gr));
+#pragma unroll
+            for (int k = K0; k < K1; k++) {
+                const int rem = (tid + k * NT) & (N - 1);
+                dst[k] = ldg(gb + off(rem >> ll, 0, rem & (L - 1)));
+            }
+        };
+        using KB = std::integral_constant<int, 0>;
+        using KE = std::integral_constant<int, EPT>;
+        if (r0 < r1) {
+            fetch_row(up, r0 - 1, KB{}, KE{});
+            fetch_row(rowa, r0, KB{}, KE{});
+            fetch_row(rowb, r0 + 1, KB{}, KE{});
+        }
+        __syncthreads();  // tables
+        auto row_step = [&](int gr, C* x, C* y) __attribute__((always_inline)) {
+            asm volatile("" : "+v"(tid));  // per-row addresses, not hoisted (registers)
+            // stage row gr, then its Laplacian over it (rows gr -+ 1 from
+            // registers); `up` becomes row gr for the next row
+#pragma unroll
+            for (int k = 0; k < EPT; k++) lds[fidx<T, LN>(tid + k * NT)] = x[k];
+            __syncthreads();
+            C lap[EPT];
+#pragma unroll
+            for (int k = 0; k < EPT; k++) {
+                lap[k] = row_laplacian<T, LVEC, LM, LN>(lds, tid + k * NT, up[k], y[k], x[k], scale, apply_scale != 0);
+                up[k] = x[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < EPT; k++) lds[fidx<T, LN>(tid + k * NT)] = lap[k];
+            __syncthreads();
+            const bool more = gr + 2 <= r1;  // row r1 is the chunk's last "below"
+            auto hook = [&](auto hc) {
+                constexpr int h = decltype(hc)::value;
+                if constexpr (h < HN) {
+                    constexpr int K0 = h * EPT / HN, K1 = (h + 1) * EPT / HN;
+                    if constexpr (K1 > K0) {
+                        if (more) fetch_row(x, gr + 2, std::integral_constant<int, K0>{}, std::integral_constant<int, K1>{});
+                    }
+                }
+            };
+            hook(std::integral_constant<int, 0>{});
+            transform_group(line_of(gr), hook);
+            __syncthreads();  // the next row's staging overwrites LDS
+        };
+        for (int gr = r0; gr < r1; gr += 2) {
+            row_step(gr, rowa, rowb);
+            if (gr + 1 < r1) row_step(gr + 1, rowb, rowa);
+        }
+        return;
+    }
+
+    int grp = blockIdx.x;
+    int it = 0;
+    if (grp < groups) {
+        fetch(grp);
+        if constexpr (MF) group_max(0);
+    }
+    __syncthreads();  // tables, maxima
+    for (; grp < groups; grp += gridDim.x, ++it) {
+        // The compiler hoists every per-thread global and LDS address out of
+        // this loop and keeps them live.  An opaque tid per iteration makes
+        // them recomputed per group instead, where that was measured faster:
+        //  * FP64, N = 8192: the hoisted addresses spill 6 to 87 VGPRs at the
+        //    256-VGPR limit; recomputed, 154-163 VGPRs and no spills (the
+        //    pass 0.562 -> 0.467 ms);
+        //  * FP32 vector ALU, several lines per group: 228 -> 102-120 VGPRs,
+        //    four 256-thread workgroups per CU instead of two (3D N = 512:
+        //    0.413 -> 0.401 ms).
+        // Elsewhere it only costs (FP64 3D N = 512: 0.725 -> 0.835 ms).
+        constexpr bool kRecompute = (sizeof(T) == 8 && LN >= 13) || (sizeof(T) == 4 && !MF && LLPI > 0);
+        if constexpr (kRecompute) asm volatile("" : "+v"(tid));
+        if constexpr (MF) {
+            float m = 0.0f;
+#pragma unroll
+            for (int v = 0; v < NW; v++) m = fmaxf(m, wmax[(it & 1) * 16 + v]);
+            int ex;
+            (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+            // a group of FP32 denormals or tiny normals would need a scale
+            // beyond 2^127: clamp, so s0 stays finite (its largest value
+            // then lands below 2^7, which costs no range)
+            ex = ex < MFMA_SCALE_LOG2 - 127 ? MFMA_SCALE_LOG2 - 127 : ex;
+            s0 = ldexpf(1.0f, MFMA_SCALE_LOG2 - ex);
+            inv_s0 = ldexpf(1.0f, ex - MFMA_SCALE_LOG2);
+        }
+        stage();
+        __syncthreads();
+        const int nxt = grp + gridDim.x;
+        // The next group's loads are spread over the pass boundaries (HN
+        // points: after staging and after every LDS pass before the last); a
+        // burst of EPT loads per thread stalls instruction issue.  Measured:
+        //  * FP32 vector ALU: always (N = 8192: 0.234 -> 0.213 ms);
+        //  * one-line groups (N >= 4096) of FP64 (N = 8192: 0.466 -> 0.401
+        //    ms) and MFMA (0.289 -> 0.260 ms);
+        //  * not the multi-line groups of FP64 or MFMA, which were slower
+        //    spread (3D N = 512: FP64 0.725 -> 0.871 ms, MFMA 0.456 -> 0.470).
+        // MF's maxima of the next group are taken after the last pass, when
+        // all of its loads were issued.
+        auto hook = [&](auto hc) {
+            constexpr int h = decltype(hc)::value;
+            if constexpr (SPREAD && h < HN) {
+                constexpr int K0 = h * EPT / HN, K1 = (h + 1) * EPT / HN;
+                if constexpr (K1 > K0) {
+                    if (nxt < groups)
+                        fetch_part(std::integral_constant<int, K0>{}, std::integral_constant<int, K1>{}, nxt);
+                }
+            }
+        };
+        if constexpr (SPREAD) hook(std::integral_constant<int, 0>{});
+        else if (nxt < groups) fetch(nxt);
+
+        transform_group(grp, hook);
         if constexpr (MF) {
             if (nxt < groups) group_max((it + 1) & 1);
         }
@@ -803,50 +823,43 @@ int lines_per_iter_log2(int n) {
 }
 
 // LDS bytes of the specialised kernel for element size `es`
-size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false, bool lapy = false) {
+size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false) {
     const int n = vec * m;
-    const int lpi = 1 << lines_per_iter_log2(n);
-    const int e = n * lpi;
+    const int e = n << lines_per_iter_log2(n);
     // the two per-pass tables, then W_N (or the per-thread PRE table of the
     // same size) or the two-level pair
     const int tab = vec + m + (fulltab ? n : 64 + (n / 64 > 0 ? n / 64 : 1));
     const int shift = n >= 4096 ? (es == 16 ? 6 : 4) : 5;
     const int padded = e + (e >> shift) + 1;
-    // MFMA weights and maxima; LAPY: the per-line sigma after the maxima
-    const size_t extra = mf || lapy ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0;
-    return es * (padded * (mf ? 2 : 1) + tab) + extra + (lapy ? (size_t)lpi * es / 2 : 0);
+    return es * (padded * (mf ? 2 : 1) + tab) + (mf ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0);
 }
 
 }  // namespace
 
-// The full W_N table is chosen when it fits LDS both with and without the
-// per-line sigma of the fused Laplacian, so that both forms of a forward pass
-// share one table layout and launch shape.
 template <typename T>
 bool fast_line_fulltab(int vec, int m, bool mf = false) {
-    return fast_lds(sizeof(Cx<T>), vec, m, true, mf, true) <= kLdsLimit;
+    return fast_lds(sizeof(Cx<T>), vec, m, true, mf) <= kLdsLimit;
 }
 
-template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPY>
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPX>
 static const void* fast_fn() {
-    return (const void*)&line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPY>;
+    return (const void*)&line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPX>;
 }
 
-template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPY>
+template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPX>
 static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a,
-                                   const Cx<T>* tables, int batch, double scale, int grid_x, const T* lap_sigma,
-                                   hipStream_t stream) {
+                                   const Cx<T>* tables, int batch, double scale, int grid_x, hipStream_t stream) {
     constexpr int LN = LVEC + LM, N = 1 << LN, E = N << LLPI;
-    const size_t lds = fast_lds(sizeof(Cx<T>), 1 << LVEC, 1 << LM, FULLTAB, MF, LAPY);
-    const void* fn = fast_fn<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPY>();
+    const size_t lds = fast_lds(sizeof(Cx<T>), 1 << LVEC, 1 << LM, FULLTAB, MF);
+    const void* fn = fast_fn<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPX>();
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int groups = (int)(g.field / N) >> LLPI;
     const dim3 grid((unsigned)(grid_x < groups ? grid_x : groups), (unsigned)batch);
-    hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPY>), grid, dim3(E / EPT), lds, stream,
-                       in, out, g, a, tables, (T)scale, scale != 1.0 ? 1 : 0, groups, lap_sigma);
+    hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPX>), grid, dim3(E / EPT), lds, stream,
+                       in, out, g, a, tables, (T)scale, scale != 1.0 ? 1 : 0, groups);
     return hipGetLastError();
 }
 
@@ -857,15 +870,26 @@ static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, co
 template <typename T, bool MF>
 constexpr bool kBuildTwoLevel = !(sizeof(T) == 4 && !MF);
 
+// The fused Laplacian + x pass (LAPX) is built for FP32 vector-ALU one-line
+// groups (N >= 4096); its register sets leave no room in FP64 or beside the
+// MFMA operands.
+template <typename T, bool MF, int V, int MM>
+constexpr bool kBuildLapx = sizeof(T) == 4 && !MF && V + MM >= 12;
+
+template <typename T, bool MF>
+bool lapx_built(int vec, int m) {
+    return sizeof(T) == 4 && !MF && vec * m >= 4096;
+}
+
 template <typename T, bool MF, typename Op>
-static auto with_fast(int vec, int m, bool fwd, bool lapy, Op&& op) {
-#define ZM_LF2(V, MM, F, FT, LY)                                                                          \
-    if constexpr (FT || kBuildTwoLevel<T, MF>) {                                                          \
-        if (vec == (1 << V) && m == (1 << MM) && fwd == F && lapy == LY &&                                \
+static auto with_fast(int vec, int m, bool fwd, bool lapx, Op&& op) {
+#define ZM_LF2(V, MM, F, FT, LX)                                                                          \
+    if constexpr ((FT || kBuildTwoLevel<T, MF>) && (!LX || (F && kBuildLapx<T, MF, V, MM>))) {        \
+        if (vec == (1 << V) && m == (1 << MM) && fwd == F && lapx == LX &&                                \
             fast_line_fulltab<T>(vec, m, MF) == FT)                                                       \
-            return op.template run<V, MM, ((V + MM) >= 12 ? 0 : 12 - (V + MM)), F, FT, MF, LY>();         \
+            return op.template run<V, MM, ((V + MM) >= 12 ? 0 : 12 - (V + MM)), F, FT, MF, LX>();         \
     }
-#define ZM_LF1(V, MM, F, FT) ZM_LF2(V, MM, F, FT, false) if constexpr (F) { ZM_LF2(V, MM, F, FT, true) }
+#define ZM_LF1(V, MM, F, FT) ZM_LF2(V, MM, F, FT, false) ZM_LF2(V, MM, F, FT, true)
 #define ZM_LF(V, MM) ZM_LF1(V, MM, true, true) ZM_LF1(V, MM, true, false) ZM_LF1(V, MM, false, true) ZM_LF1(V, MM, false, false)
     ZM_LF(4, 4) ZM_LF(4, 5) ZM_LF(4, 6) ZM_LF(4, 7)
     ZM_LF(5, 4) ZM_LF(5, 5) ZM_LF(5, 6) ZM_LF(5, 7)
@@ -887,12 +911,10 @@ struct LaunchOp {
     int batch;
     double scale;
     int grid_x;
-    const T* lap_sigma;
     hipStream_t stream;
-    template <int V, int MM, int LL, bool F, bool FT, bool MF, bool LY>
+    template <int V, int MM, int LL, bool F, bool FT, bool MF, bool LX>
     hipError_t run() {
-        return launch_fast_inst<T, V, MM, LL, F, FT, MF, LY>(in, out, g, a, tables, batch, scale, grid_x, lap_sigma,
-                                                             stream);
+        return launch_fast_inst<T, V, MM, LL, F, FT, MF, LX>(in, out, g, a, tables, batch, scale, grid_x, stream);
     }
     hipError_t none() { return hipErrorInvalidValue; }
 };
@@ -900,10 +922,10 @@ struct LaunchOp {
 template <typename T>
 struct OccOp {
     int vec, m;
-    template <int V, int MM, int LL, bool F, bool FT, bool MF, bool LY>
+    template <int V, int MM, int LL, bool F, bool FT, bool MF, bool LX>
     int run() {
-        const void* fn = fast_fn<T, V, MM, LL, F, FT, MF, LY>();
-        const size_t lds = fast_lds(sizeof(Cx<T>), vec, m, FT, MF, LY);
+        const void* fn = fast_fn<T, V, MM, LL, F, FT, MF, LX>();
+        const size_t lds = fast_lds(sizeof(Cx<T>), vec, m, FT, MF);
         const int nthreads = ((1 << (V + MM)) << LL) / EPT;
         if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int blocks = 0;
@@ -918,30 +940,34 @@ bool fast_line_supported(int vec, int m) {
     const bool okv = vec == 16 || vec == 32 || vec == 64 || vec == 128;
     const bool okm = m == 16 || m == 32 || m == 64 || m == 128;
     return okv && okm && (int64_t)vec * m <= 8192 &&
-           fast_lds(sizeof(Cx<T>), vec, m, false, sizeof(T) == 4, true) <= kLdsLimit;
+           fast_lds(sizeof(Cx<T>), vec, m, false, sizeof(T) == 4) <= kLdsLimit;
 }
 
 template <typename T>
-int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapy) {
+int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapx) {
+    if (lapx && !(fwd && (mf ? lapx_built<T, true>(vec, m) : lapx_built<T, false>(vec, m)))) return 0;
     if constexpr (sizeof(T) == 4) {
-        if (mf) return with_fast<T, true>(vec, m, fwd, lapy, OccOp<T>{vec, m});
+        if (mf) return with_fast<T, true>(vec, m, fwd, lapx, OccOp<T>{vec, m});
     }
     if (mf) return 0;
-    return with_fast<T, false>(vec, m, fwd, lapy, OccOp<T>{vec, m});
+    return with_fast<T, false>(vec, m, fwd, lapx, OccOp<T>{vec, m});
+}
+
+template <typename T>
+bool fast_line_lapx(int vec, int m, bool mf) {
+    return fast_line_supported<T>(vec, m) && (mf ? lapx_built<T, true>(vec, m) : lapx_built<T, false>(vec, m));
 }
 
 template <typename T>
 hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
-                            int batch, bool fwd, double scale, int grid_x, bool mf, const T* lap_sigma,
-                            hipStream_t stream) {
-    const bool lapy = lap_sigma != nullptr;
-    if (lapy && !fwd) return hipErrorInvalidValue;  // the fused Laplacian precedes a forward pass only
-    const LaunchOp<T> op{in, out, g, a, tables, batch, scale, grid_x, lap_sigma, stream};
+                            int batch, bool fwd, double scale, int grid_x, bool mf, bool lapx, hipStream_t stream) {
+    if (lapx && !(fwd && g.dims == 2 && fast_line_lapx<T>(g.vec, g.m, mf))) return hipErrorInvalidValue;
+    const LaunchOp<T> op{in, out, g, a, tables, batch, scale, grid_x, stream};
     if constexpr (sizeof(T) == 4) {
-        if (mf) return with_fast<T, true>(g.vec, g.m, fwd, lapy, LaunchOp<T>(op));
+        if (mf) return with_fast<T, true>(g.vec, g.m, fwd, lapx, LaunchOp<T>(op));
     }
     if (mf) return hipErrorInvalidValue;  // the split-precision passes are FP32 only
-    return with_fast<T, false>(g.vec, g.m, fwd, lapy, LaunchOp<T>(op));
+    return with_fast<T, false>(g.vec, g.m, fwd, lapx, LaunchOp<T>(op));
 }
 
 // FP64 twiddle tables of a line (N = vec*m, S = max(vec, m)), rounded for
@@ -987,9 +1013,11 @@ template bool fast_line_supported<float>(int, int);
 template bool fast_line_supported<double>(int, int);
 template int fast_line_occupancy<float>(int, int, bool, bool, bool);
 template int fast_line_occupancy<double>(int, int, bool, bool, bool);
+template bool fast_line_lapx<float>(int, int, bool);
+template bool fast_line_lapx<double>(int, int, bool);
 template hipError_t launch_line_fast<float>(const float2*, float2*, const Geo&, const LineArgs&, const float2*, int,
-                                            bool, double, int, bool, const float*, hipStream_t);
+                                            bool, double, int, bool, bool, hipStream_t);
 template hipError_t launch_line_fast<double>(const double2*, double2*, const Geo&, const LineArgs&, const double2*,
-                                             int, bool, double, int, bool, const double*, hipStream_t);
+                                             int, bool, double, int, bool, bool, hipStream_t);
 
 }  // namespace zm

@@ -35,8 +35,8 @@ struct zm_plan {
     bool fast;
     double2* ltab64;  // per-line twiddle tables (W_S, W_N two-level)
     float2* ltab32;
-    int fast_grid_fwd, fast_grid_inv, fast_grid_lap;  // persistent grids (the last: fused Laplacian pass)
-    void* lap_sigma;  // per-line symbol of the fused Laplacian's last pass (T per line)
+    int fast_grid_fwd, fast_grid_inv;
+    int fast_grid_lapx;  // the fused Laplacian + x pass's grid (0: not built for this plan)
     // per-stage timing (zm_timing_*)
     bool timing;
     struct Timed {
@@ -162,36 +162,6 @@ int ipow(int a, int e) {
 constexpr int kLineTarget = 4096;
 constexpr int kStageTarget = 4096;
 constexpr size_t kMaxLds = 160 * 1024;
-
-}  // namespace
-
-namespace {
-
-// Symbol of the fused Laplacian (zm_laplace_forward) for every line of the
-// forward transform's last direction pass: the sum over the directions
-// already transformed of -4 sin^2(pi k / N), the exact symbol of the periodic
-// second difference, with k from the reciprocal maps (zmodel/utils.h:254-320).
-// 2D, y pass: line (o, q) of exchange group o = cid, local row q = lr has
-// kx = ct(cid L + lp(lr)).  3D, z pass: o = rid P + cid, q = ld L + lr give
-// kx (cid, lr) and ky (rid, ld).
-template <typename T>
-std::vector<T> laplace_sigma_table(const zm_shape& sh, const zm_dims& d) {
-    const zm::MapShape ms = map_shape(&sh);
-    const int64_t lpg = d.local / d.l, lines = d.field / sh.n;
-    std::vector<T> t((size_t)lines);
-    auto sig = [&](int k) {
-        const double h = std::sin(M_PI * (double)k / (double)sh.n);
-        return -4.0 * h * h;
-    };
-    for (int64_t lam = 0; lam < lines; ++lam) {
-        const int o = (int)(lam / lpg), q = (int)(lam % lpg);
-        double acc;
-        if (sh.dims == 2) acc = sig(zm::recip_to_freq(ms, o, q));
-        else acc = sig(zm::recip_to_freq(ms, o % sh.p, q % d.l)) + sig(zm::recip_to_freq(ms, o / sh.p, q / d.l));
-        t[(size_t)lam] = (T)acc;
-    }
-    return t;
-}
 
 }  // namespace
 
@@ -327,23 +297,17 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
                                 : zm::fast_line_occupancy<float>(d.vec, d.m, true, mfma, false);
             const int oi = fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, false, false, false)
                                 : zm::fast_line_occupancy<float>(d.vec, d.m, false, mfma, false);
-            const int ol = fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, true, false, true)
-                                : zm::fast_line_occupancy<float>(d.vec, d.m, true, mfma, true);
-            if (e == hipSuccess && of > 0 && oi > 0 && ol > 0) {
+            // the fused Laplacian + x pass: 2D only (it needs the rows above
+            // and below, not the planes before and after)
+            const bool lx = shape->dims == 2 && (fp64 ? zm::fast_line_lapx<double>(d.vec, d.m, false)
+                                                      : zm::fast_line_lapx<float>(d.vec, d.m, mfma));
+            const int ol = !lx ? 0 : fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, true, false, true)
+                                          : zm::fast_line_occupancy<float>(d.vec, d.m, true, mfma, true);
+            if (e == hipSuccess && of > 0 && oi > 0) {
                 p->fast = true;
                 p->fast_grid_fwd = of * cus;
                 p->fast_grid_inv = oi * cus;
-                p->fast_grid_lap = ol * cus;
-                // the fused Laplacian's per-line symbols
-                if (fp64) {
-                    const std::vector<double> t = laplace_sigma_table<double>(*shape, d);
-                    e = hipMalloc(&p->lap_sigma, t.size() * sizeof(double));
-                    if (e == hipSuccess) e = hipMemcpy(p->lap_sigma, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
-                } else {
-                    const std::vector<float> t = laplace_sigma_table<float>(*shape, d);
-                    e = hipMalloc(&p->lap_sigma, t.size() * sizeof(float));
-                    if (e == hipSuccess) e = hipMemcpy(p->lap_sigma, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
-                }
+                p->fast_grid_lapx = ol * cus;
             }
         }
     }
@@ -353,7 +317,6 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
         (void)hipFree(p->tw32);
         if (p->ltab64) (void)hipFree(p->ltab64);
         if (p->ltab32) (void)hipFree(p->ltab32);
-        if (p->lap_sigma) (void)hipFree(p->lap_sigma);
         if (p->own_scratch) (void)hipFree(p->own_scratch);
         delete p;
         return e != hipSuccess ? ZM_ERR_DEVICE : ZM_ERR_UNSUPPORTED;
@@ -369,7 +332,6 @@ zm_status zm_plan_destroy(zm_plan* p) {
     if (p->tw32 && hipFree(p->tw32) != hipSuccess) st = ZM_ERR_DEVICE;
     if (p->ltab64 && hipFree(p->ltab64) != hipSuccess) st = ZM_ERR_DEVICE;
     if (p->ltab32 && hipFree(p->ltab32) != hipSuccess) st = ZM_ERR_DEVICE;
-    if (p->lap_sigma && hipFree(p->lap_sigma) != hipSuccess) st = ZM_ERR_DEVICE;
     if (p->own_scratch && hipFree(p->own_scratch) != hipSuccess) st = ZM_ERR_DEVICE;
     drop_timing(p);
     delete p;
@@ -444,19 +406,18 @@ struct Exec {
     const zm::Cx<T>* line_tables() const {
         return p->precision == ZM_FP64 ? (const zm::Cx<T>*)p->ltab64 : (const zm::Cx<T>*)p->ltab32;
     }
-    // lapy: the fused Laplacian (forward, last direction, specialised pass
-    // only), `scale` being the Laplacian's scale
-    zm_status line(const void* in, void* out, int axis, bool fwd, double scale, bool lapy = false) const {
+    // lapx: the x pass of zm_laplace_forward (fast_grid_lapx != 0 only),
+    // transforming `scale` times the input's Laplacian
+    zm_status line(const void* in, void* out, int axis, bool fwd, double scale, bool lapx = false) const {
         zm::LineArgs a{axis_stride(p, axis), (int)(p->dims.local / p->dims.l), p->lpb};
         if (p->fast && compute) {
-            const int grid = lapy ? p->fast_grid_lap : fwd ? p->fast_grid_fwd : p->fast_grid_inv;
-            return launch_status(timed_launch(p, ZM_STAGE_LINE, st, [&] {
+            const int grid = lapx ? p->fast_grid_lapx : fwd ? p->fast_grid_fwd : p->fast_grid_inv;
+            return launch_status(timed_launch(p, lapx ? ZM_STAGE_LINE_LAPLACE : ZM_STAGE_LINE, st, [&] {
                 return zm::launch_line_fast<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, a, line_tables(),
-                                               p->batch, fwd, scale, grid, p->precision == ZM_FP32_MFMA,
-                                               lapy ? (const T*)p->lap_sigma : nullptr, st);
+                                               p->batch, fwd, scale, grid, p->precision == ZM_FP32_MFMA, lapx, st);
             }, record));
         }
-        if (lapy) return ZM_ERR_UNSUPPORTED;
+        if (lapx) return ZM_ERR_UNSUPPORTED;
         return launch_status(timed_launch(p, ZM_STAGE_LINE, st, [&] {
             return zm::launch_line<T>((const zm::Cx<T>*)in, (zm::Cx<T>*)out, p->geo, a, tw, p->batch, fwd, compute,
                                       scale, st);
@@ -518,28 +479,23 @@ struct Exec {
         return ZM_OK;
     }
 
-    // The reference's Laplacian followed by forward_fft (zmodel/main.cu:50-63)
-    // as one chain: the earlier directions transform the input itself, and
-    // the last direction's pass applies the Laplacian (line_laplacian in the
-    // line pass) before its DFTs; the specialised fused path only.
+    // The reference's laplace followed by forward_fft (zmodel/main.cu:50-63).
+    // With the fused pass the Laplacian is formed inside the x pass (three
+    // sweeps instead of four); otherwise the stencil writes `out` and the
+    // forward chain runs in place on it (its first pass reads `out` whole
+    // before the repack writes it).  Either way the result is bit for bit
+    // zm_forward(zm_laplace(in)).
     zm_status laplace_forward(const void* in, void* out, void* s, double lap_scale) const {
         if (p->two_stage) return ZM_ERR_UNSUPPORTED;
-        const bool d3 = p->shape.dims == 3;
-        if (!p->fast) {
-            // the generic line pass has no fused Laplacian: the stencil into
-            // `out`, then the forward chain in place (its first pass reads
-            // `out` whole before the repack writes it)
-            ZM_TRY(launch_status(timed_launch(p, ZM_STAGE_LAPLACE, st, [&] {
-                return zm::launch_stencil<T>(in, out, p->geo, p->batch, zm::OP_LAPLACE, true, lap_scale, false, st);
-            }, record)));
-            return forward(out, out, s);
+        if (p->fast_grid_lapx > 0) {
+            ZM_TRY(line(in, s, 0, true, lap_scale, true));
+            ZM_TRY(repack(s, out, REPACK_2D, true));
+            return line(out, out, 1, true, 1.0);
         }
-        ZM_TRY(line(in, s, 0, true, 1.0));
-        ZM_TRY(repack(s, out, d3 ? REPACK_XY : REPACK_2D, true));
-        if (!d3) return line(out, out, 1, true, lap_scale, true);
-        ZM_TRY(line(out, out, 1, true, 1.0));
-        ZM_TRY(repack(out, s, REPACK_YZ, true));
-        return line(s, out, 2, true, lap_scale, true);
+        ZM_TRY(launch_status(timed_launch(p, ZM_STAGE_LAPLACE, st, [&] {
+            return zm::launch_stencil<T>(in, out, p->geo, p->batch, zm::OP_LAPLACE, true, lap_scale, false, st);
+        }, record)));
+        return forward(out, out, s);
     }
 
     // inverse_fft / inverse_fft_3d (zmodel/transforms/fft.cu:409-514,

@@ -234,9 +234,9 @@ class Plan:
         return out
 
     def laplace_forward(self, x, scale: float = 1.0, out=None, scratch=None, stream=None):
-        """Forward transform of scale * Laplacian(x) in one chain (the Laplacian
-        + forward_fft of zmodel/main.cu:50-63; equal to forward(laplace(x))
-        up to rounding)."""
+        """Forward transform of scale * Laplacian(x): the Laplacian + forward_fft
+        of zmodel/main.cu:50-63, bit for bit forward(laplace(x, scale)); the
+        Laplacian is formed inside the x pass where the plan has that pass."""
         out = self._out(x, out)
         sp = self._scratch(scratch, x, out)
         L.check(L._lib().zm_laplace_forward(self._h, _ptr(x), _ptr(out), float(scale), sp, _stream(stream)),

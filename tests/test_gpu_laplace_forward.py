@@ -1,17 +1,14 @@
 """zm_laplace_forward: the reference's Laplacian followed by forward_fft
-(zmodel/main.cu:50-63) as one chain, the Laplacian formed inside the last
-direction's line pass.
+(zmodel/main.cu:50-63) as one call.
 
-* FP64 against the two-call sequence zm_laplace + zm_forward on the GPU, and
-  against the oracle's restatement of laplace.cu:73-169 / laplace3d.cu:90-225
-  followed by fft.cu:199-339 / fft3d.cu:234-412: <= 1e-13 relative max error
-  (the fused form is not bit-identical: it applies the second difference of
-  the transformed directions as its exact symbol -4 sin^2(pi k / N)).
-* FP32 and the split-precision MFMA path against the FP64 two-call sequence on
-  the same FP32-rounded input: <= 1e-6 relative L2, <= 1e-5 relative max.
-* Shapes cover every (VEC, M) family of the specialised pass that the 2D and
-  3D benchmark shapes use, a batch of two, and a plan without the
-  specialised pass (the two-call fallback).
+On 2D FP32 plans with the specialised pass and N >= 4096 the Laplacian is
+formed inside the forward transform's x pass (rows above and below from
+registers, x neighbours from the row in LDS); elsewhere the call runs the
+stencil and then the transform.  Either way the result must be bit for bit
+zm_forward(zm_laplace(x, scale)), whose own parity with the oracle
+(laplace.cu:73-169, laplace3d.cu:90-225, fft.cu:199-339) the other GPU tests
+hold.  Shapes cover both paths, the C2 shape on its smooth mode field, a
+batch of two, and the three precisions.
 """
 import sys
 
@@ -19,12 +16,12 @@ import numpy as np
 import pytest
 
 from conftest import has_gpu
-from helpers import FP32_REL_MAX, MFMA_REL_MAX
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a GPU")]
 
-SHAPES = [(2, 256, 2, 8), (2, 1024, 2, 16), (2, 2048, 4, 8), (2, 4096, 4, 16), (2, 8192, 4, 32), (2, 8192, 2, 32),
-          (3, 256, 2, 8), (3, 512, 2, 16)]
+# (dims, n, p, b, fused x pass expected in FP32)
+SHAPES = [(2, 256, 2, 8, False), (2, 1024, 2, 16, False), (2, 4096, 4, 16, True), (2, 4096, 2, 16, True),
+          (2, 8192, 4, 32, True), (2, 8192, 2, 32, True), (3, 256, 2, 8, False), (3, 512, 2, 16, False)]
 
 
 def note(msg):
@@ -42,85 +39,72 @@ def _field(dims, n, seed, batch=1):
     return torch.complex(re, im)
 
 
-def _rel(a, ref):
+def _same_bits(a, b):
     import torch
-    d = a.to(torch.complex128) - ref.to(torch.complex128)
-    return (float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref)),
-            float(d.abs().max() / ref.abs().max()))
+    return bool(torch.equal(torch.view_as_real(a).contiguous().view(torch.int8),
+                            torch.view_as_real(b).contiguous().view(torch.int8)))
 
 
-@pytest.mark.parametrize("dims,n,p,b", SHAPES)
-def test_fused_matches_two_calls(dims, n, p, b):
-    from stencilfft_amd import Plan
-    note(f"laplace_forward {dims}D N={n} P={p} B={b}")
-    scale = float(n * n)
-    p64 = Plan(dims, n, p, b, precision="fp64")
-    x = _field(dims, n, 900 + n + b)
-    want = p64.forward(p64.laplace(x, scale))
-    got = p64.laplace_forward(x, scale)
-    l2, mx = _rel(got, want)
-    note(f"  fp64 vs laplace + forward: rel L2 {l2:.2e}, rel max {mx:.2e}")
-    assert mx < 1e-13
-    x32 = x.to(__import__("torch").complex64)
-    want32 = p64.forward(p64.laplace(x32.to(x.dtype), scale))
-    for prec, bar in (("fp32", FP32_REL_MAX), ("fp32_mfma", MFMA_REL_MAX)):
-        pl = Plan(dims, n, p, b, precision=prec)
-        l2, mx = _rel(pl.laplace_forward(x32, scale), want32)
-        note(f"  {prec} vs fp64: rel L2 {l2:.2e}, rel max {mx:.2e}")
-        assert l2 <= 1e-6 and mx <= bar
-        # the same against the two-call FP32 path, whose error it should match
-        l2b, _ = _rel(pl.forward(pl.laplace(x32, scale)), want32)
-        assert l2 <= 2.0 * l2b + 1e-7
+def _stages(pl, fn):
+    import torch
+    pl.timing_reset()
+    pl.timing(True)
+    out = fn()
+    pl.timing(False)
+    torch.cuda.synchronize()
+    st = pl.timing_read()
+    pl.timing_reset()
+    return out, st
 
 
-def test_fused_batch_and_round_trip():
+@pytest.mark.parametrize("dims,n,p,b,fused", SHAPES)
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32_mfma"])
+def test_laplace_forward_is_laplace_then_forward(dims, n, p, b, fused, precision):
     import torch
     from stencilfft_amd import Plan
-    dims, n, p, b = 2, 2048, 4, 8
-    note(f"laplace_forward batch 2, round trip {dims}D N={n}")
+    note(f"laplace_forward {precision} {dims}D N={n} P={p} B={b}")
+    pl = Plan(dims, n, p, b, precision=precision)
+    x = _field(dims, n, 900 + n + b).to(pl.complex_dtype)
     scale = float(n * n)
-    pl = Plan(dims, n, p, b, precision="fp64", batch=2)
-    x = _field(dims, n, 77, batch=2)
-    got = pl.laplace_forward(x, scale)
     want = pl.forward(pl.laplace(x, scale))
-    assert _rel(got, want)[1] < 1e-13
-    z = pl.inverse(got, 1.0 / n ** 2)
-    assert _rel(z, pl.laplace(x, scale))[1] < 1e-13
+    got, st = _stages(pl, lambda: pl.laplace_forward(x, scale))
+    assert _same_bits(got, want)
+    expect_fused = fused and precision == "fp32"
+    assert ("laplace" not in st) == expect_fused, st
+    if expect_fused:
+        assert st["line_laplace"][0] == 1 and st["line"][0] == 1 and st["repack"][0] == 1
     del torch
 
 
-@pytest.mark.parametrize("dims,n,p,b", [(2, 64, 4, 2), (3, 32, 2, 4)])
-def test_fused_against_oracle_and_fallback(oracle_mod, dims, n, p, b):
-    """Plans without the specialised pass take the two-call fallback; both
-    paths against the oracle's Laplacian + forward transform."""
+def test_c2_smooth_field_and_batch():
+    """The benchmark's field (256 smooth modes plus 1e-3 noise) at 8192^2,
+    P=4, B=32, in a batch of two with a second field."""
     import torch
     from stencilfft_amd import Plan
-    note(f"laplace_forward vs oracle {dims}D N={n} P={p} B={b}")
+    from stencilfft_amd.fields import mode_field_natural
+    n, p, b = 8192, 4, 32
+    note("laplace_forward C2 smooth field, batch 2")
+    p64 = Plan(2, n, p, b, precision="fp64")
+    nat = mode_field_natural(n, 2, 256, n // 8, 8192, noise=1e-3)
+    x = p64.natural_to_packed(nat).to(torch.complex64)
+    del nat, p64
+    pl = Plan(2, n, p, b, precision="fp32", batch=2)
+    xb = torch.cat([x, _field(2, n, 3).to(torch.complex64)])
+    scale = float(n * n)
+    assert _same_bits(pl.laplace_forward(xb, scale), pl.forward(pl.laplace(xb, scale)))
+
+
+def test_fallback_against_oracle(oracle_mod):
+    import torch
+    from stencilfft_amd import Plan
+    dims, n, p, b = 2, 64, 4, 2
+    note(f"laplace_forward vs oracle {dims}D N={n}")
     s = oracle_mod.Shape(dims, n, p, b)
     rng = np.random.default_rng(n)
-    nat = rng.uniform(-1, 1, (n,) * dims) + 1j * rng.uniform(-1, 1, (n,) * dims)
-    x = s.natural_to_real(nat)
-    want = s.forward_fft(s.laplace(x) * 3.0)
-    pl = Plan(dims, n, p, b, precision="fp64")
-    got = pl.laplace_forward(torch.from_numpy(x).cuda(), 3.0).cpu().numpy()
-    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-13
-
-
-def test_fused_against_oracle_on_the_fast_path(oracle_mod):
-    import torch
-    from stencilfft_amd import Plan
-    dims, n, p, b = 2, 256, 2, 8
-    note(f"laplace_forward vs oracle (specialised pass) {dims}D N={n}")
-    s = oracle_mod.Shape(dims, n, p, b)
-    rng = np.random.default_rng(5)
     x = s.natural_to_real(rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)))
     pl = Plan(dims, n, p, b, precision="fp64")
-    from stencilfft_amd import _lib as L
-    path = __import__("ctypes").c_uint()
-    L.check(L._lib().zm_plan_path(pl.handle, __import__("ctypes").byref(path)))
-    assert path.value & L.ZM_PATH_FAST_LINE
-    got = pl.laplace_forward(torch.from_numpy(x).cuda(), float(n * n)).cpu().numpy()
-    want = s.forward_fft(s.laplace(x) * float(n * n))
+    got = pl.laplace_forward(torch.from_numpy(x).cuda(), 3.0).cpu().numpy()
+    want = s.forward_fft(s.laplace(x) * 3.0)
     assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-13
 
 
