@@ -92,6 +92,10 @@ __device__ __forceinline__ int pre_exp(int u, int r) {
 //           (the M-point DFTs); lanes run along k1.
 // PRE: multiply element i of transform bt by W_N^(i * kk(bt)) first.
 // TOGLOBAL: instead of writing LDS, hand each output to `sink`.
+// FROMREG: the inputs are this thread's registers rin[k*R + r] (loaded from
+// HBM straight into this pass's order) instead of LDS; the writes need no
+// barrier before them, the previous group's last reads of LDS having ended
+// at the caller's barrier.
 // OOP: read `src`, write `dst` (a different LDS buffer) as soon as each
 // transform is done, one barrier at the end; otherwise src == dst and all
 // reads finish (barrier) before the writes start.
@@ -101,9 +105,9 @@ __device__ __forceinline__ int pre_exp(int u, int r) {
 // per-pass table, W_{NS R}^(jm r) at r*NS + jm (jm runs along the lanes, so
 // both reads are free of bank conflicts).
 template <typename T, int LN_T, int NS, int R, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool TOGLOBAL, bool FULLTAB, bool OOP, bool PTW, typename Sink>
+          bool TOGLOBAL, bool FULLTAB, bool OOP, bool PTW, bool FROMREG = false, typename Sink>
 __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx<T>* ts, const Cx<T>* ta,
-                                          const Cx<T>* tb, int tid, Sink&& sink) {
+                                          const Cx<T>* tb, int tid, Sink&& sink, const Cx<T>* rin = nullptr) {
     using C = Cx<T>;
     constexpr int n = 1 << LN_T;
     constexpr int nbf = n / R;
@@ -134,8 +138,13 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
             base = (line << LN) + k1;
             kk = k1;
         }
+        if constexpr (FROMREG) {
 #pragma unroll
-        for (int r = 0; r < R; r++) v[k][r] = src[fidx<T, LN>(base + (j + r * nbf) * ES)];
+            for (int r = 0; r < R; r++) v[k][r] = rin[k * R + r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[k][r] = src[fidx<T, LN>(base + (j + r * nbf) * ES)];
+        }
         if (PRE) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -163,7 +172,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
     if constexpr (OOP && !TOGLOBAL) {
         __syncthreads();
     } else if constexpr (!TOGLOBAL) {
-        __syncthreads();
+        if constexpr (!FROMREG) __syncthreads();
 #pragma unroll
         for (int k = 0; k < BPT; k++) {
 #pragma unroll
@@ -355,19 +364,22 @@ struct NoSink {
 // OOP the passes alternate between the buffers `cur` and `alt`; the return
 // value is the buffer that holds the result.
 template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool FULLTAB, bool MF, bool OOP, int HOFF = 0, typename Hook = NoHook>
+          bool FULLTAB, bool MF, bool OOP, int HOFF = 0, bool FROMREG = false, typename Hook = NoHook>
 __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const Cx<T>* ts, const Cx<T>* ta,
-                                                  const Cx<T>* tb, int tid, const Mfma16W* w, Hook&& hook = Hook{}) {
+                                                  const Cx<T>* tb, int tid, const Mfma16W* w, Hook&& hook = Hook{},
+                                                  const Cx<T>* rin = nullptr) {
     if constexpr (PASS < NPASS) {
         constexpr int R = fast::Radix<LN_T, PASS>::value;
         constexpr int NS = 1 << (4 * PASS);
         Cx<T>* dst = OOP ? alt : cur;
+        static_assert(!(MF && FROMREG), "the MFMA passes read LDS");
         if constexpr (MF && R == 16)
             mfma_pass16<T, LN_T, NS, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP>(
                 cur, dst, ts, ta, tb, tid, w, NoSink{});
         else
             fast_pass<T, LN_T, NS, R, LVEC, LN, LE, NT, CONTIG, FWD, PRE && PASS == 0, false, FULLTAB, OOP,
-                      kPerThreadPre<MF, FULLTAB, LN, LE>>(cur, dst, ts, ta, tb, tid, NoSink{});
+                      kPerThreadPre<MF, FULLTAB, LN, LE>, FROMREG && PASS == 0>(cur, dst, ts, ta, tb, tid, NoSink{},
+                                                                               rin);
         hook(std::integral_constant<int, HOFF + PASS + 1>{});
         return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP, HOFF>(
             dst, OOP ? cur : alt, ts, ta, tb, tid, w, hook);
@@ -389,24 +401,17 @@ __device__ __forceinline__ void last_pass(const Cx<T>* cur, const Cx<T>* ts, con
                   kPerThreadPre<MF, FULLTAB, LN, LE>>(cur, nullptr, ts, ta, tb, tid, sink);
 }
 
-// The periodic 5-point Laplacian of global row gr at element e of the line,
-// from the rows above / below (this thread's elements of rows gr -+ 1) and
-// the row itself in LDS: its x neighbours are the line's vectors m -+ 1 (line
-// coordinate x = v M + m, v the lane; at m = 0 / M - 1 the neighbouring lane's
-// other end, the reference's lane shift at the global x edge, laplace.cu:148,
-// 160).  FP64 sums in the reference's order (laplace.cu:117-167), FP32 in the
-// first-difference order of the stencil kernels, both then times `scale`: the
-// same operations as zm_laplace, so the values are bit for bit its own.
-template <typename T, int LVEC, int LM, int LN>
-__device__ __forceinline__ Cx<T> row_laplacian(const Cx<T>* row, int e, Cx<T> a, Cx<T> b, Cx<T> c, T scale,
-                                               bool apply_scale) {
+// The periodic 5-point Laplacian at one element from its four neighbours:
+// a / b the rows above / below, l / r the x neighbours (line vectors m -+ 1;
+// at m = 0 / M - 1 the neighbouring lane's other end, the reference's lane
+// shift at the global x edge, laplace.cu:148, 160).  FP64 sums in the
+// reference's order (laplace.cu:117-167), FP32 in the first-difference order
+// of the stencil kernels, both then times `scale`: the same operations as
+// zm_laplace, so the values are bit for bit its own.
+template <typename T>
+__device__ __forceinline__ Cx<T> lap5(Cx<T> a, Cx<T> b, Cx<T> c, Cx<T> l, Cx<T> r, T scale, bool apply_scale) {
 #pragma clang fp contract(off)
     using C = Cx<T>;
-    constexpr int VEC = 1 << LVEC, M = 1 << LM;
-    const int m = e >> LVEC, v = e & (VEC - 1);
-    const int il = m > 0 ? e - VEC : ((M - 1) << LVEC) + ((v - 1) & (VEC - 1));
-    const int ir = m < M - 1 ? e + VEC : ((v + 1) & (VEC - 1));
-    const C l = row[fidx<T, LN>(il)], r = row[fidx<T, LN>(ir)];
     C s;
     if constexpr (sizeof(T) == 8) {
         s = C{(T)0, (T)0};
@@ -423,6 +428,17 @@ __device__ __forceinline__ Cx<T> row_laplacian(const Cx<T>* row, int e, Cx<T> a,
     if (apply_scale) s = C{s.x * scale, s.y * scale};
     return s;
 }
+
+// Whether the group goes from HBM straight into the first radix pass's
+// registers: vector-ALU inverse passes of one-line groups, whose first
+// (M-point) pass takes runs of VEC / P contiguous elements per rank across
+// the lanes.  Measured against staging through LDS on the same box: 8192^2
+// FP32 0.211 -> 0.207 ms, FP64 0.395 -> 0.384 ms, 4096^2 FP32 unchanged.
+// Not the forward passes, whose first (VEC-point, stride VEC/16) pass would
+// take 32-byte pieces per lane group (8192^2 FP32 0.205 -> 0.291 ms), nor
+// the multi-line groups of 3D (512^3 FP32 inverse 0.416 -> 0.418 ms).
+template <typename T, bool MF, int LVEC, int LM, int LLPI, bool FWD>
+constexpr bool kDirectFirstPass = !MF && !FWD && LLPI == 0;
 
 // Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
 // iteration.  a.lpb is the number of iterations (line groups) per workgroup.
@@ -447,6 +463,10 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     // (inverse) or VEC/P-element runs (forward), i.e. partial cache lines per
     // store instruction; instead it stays in LDS and a sweep writes whole rows
     constexpr bool STAGED = LVEC == 4;
+    // DIRECT: the group is loaded from HBM straight into the order the first
+    // radix pass takes it in, which then runs from registers: no staging
+    // writes, reads or barriers
+    constexpr bool DIRECT = kDirectFirstPass<T, MF, LVEC, LM, LLPI, FWD>;
     C* lds = reinterpret_cast<C*>(smem_raw);
     C* lds2 = OOP ? lds + fast_padded<T, LN>(E) : lds;
     C* tabs = lds + fast_padded<T, LN>(E) * (OOP ? 2 : 1);
@@ -527,26 +547,53 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     auto ldg = [](const C* p) -> C { return ld_nt(p); };
     auto stg = [](C* p, C v) { st_nt(p, v); };
 
-    // register prefetch of one line group: element e = tid + k*NT
+    // DIRECT: input k of this thread's first radix pass (work item k / R0,
+    // input k % R0) is LDS-order element slot_li(k) of the group, whose
+    // global offset from the group base is li_off (the inverse of stage()'s
+    // map below)
+    auto slot_li = [&](int k) -> int {
+        if constexpr (FWD) {
+            constexpr int R0 = fast::Radix<LVEC, 0>::value, nbf = VEC / R0, LNBF = fast::Log2<nbf>::value;
+            const int u = tid + (k / R0) * NT, r = k % R0;
+            return ((u >> LNBF) << LVEC) + (u & (nbf - 1)) + r * nbf;
+        } else {
+            constexpr int R0 = fast::Radix<LM, 0>::value, nbf = M / R0, LNBF = fast::Log2<nbf>::value;
+            const int u = tid + (k / R0) * NT, r = k % R0;
+            const int rest = u >> LVEC;
+            return ((rest >> LNBF) << LN) + (u & (VEC - 1)) + ((rest & (nbf - 1)) + r * nbf) * VEC;
+        }
+    };
+    auto li_off = [&](int li) -> int {
+        const int l = li >> LN, x = li & (N - 1);
+        if constexpr (FWD) {
+            return off(x >> ll, l, x & (L - 1));
+        } else {
+            const int lo = x & (VEC - 1), hi = x >> LVEC;
+            return off(lo >> lvp, l, (hi << lvp) + (lo & (VP - 1)));
+        }
+    };
+    // register prefetch of one line group: element e = tid + k*NT (DIRECT:
+    // slot_li(k))
     C pre[EPT];
+    auto slot_off = [&](int k) -> int {
+        if constexpr (DIRECT) {
+            return li_off(slot_li(k));
+        } else {
+            const int e = tid + k * NT;
+            const int l = e >> LN, rem = e & (N - 1);
+            return off(rem >> ll, l, rem & (L - 1));
+        }
+    };
     auto fetch_part = [&](auto k0c, auto k1c, int grp) {
         constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
         const C* gb = src0 + group_base(grp);
 #pragma unroll
-        for (int k = K0; k < K1; k++) {
-            const int e = tid + k * NT;
-            const int l = e >> LN, rem = e & (N - 1);
-            pre[k] = ldg(gb + off(rem >> ll, l, rem & (L - 1)));
-        }
+        for (int k = K0; k < K1; k++) pre[k] = ldg(gb + slot_off(k));
     };
     auto fetch = [&](int grp) {
         const C* gb = src0 + group_base(grp);
 #pragma unroll
-        for (int k = 0; k < EPT; k++) {
-            const int e = tid + k * NT;
-            const int l = e >> LN, rem = e & (N - 1);
-            pre[k] = ldg(gb + off(rem >> ll, l, rem & (L - 1)));
-        }
+        for (int k = 0; k < EPT; k++) pre[k] = ldg(gb + slot_off(k));
     };
     // MF: largest |re|, |im| of the prefetched group, reduced over the wave
     // and left in maxima slot `slot` for the next stage()
@@ -582,16 +629,18 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
     constexpr int NP1 = FWD ? NPV : NPM;
     constexpr int NP2 = STAGED ? (FWD ? NPM : NPV) : ((FWD ? NPM : NPV) - 1);
     constexpr int HN = 1 + NP1 + NP2;
+    static_assert(!DIRECT || SPREAD, "the direct first pass takes the next group's loads after it");
 
     // The line group's transform, from the staged LDS data to HBM, with the
     // next group's loads spread over its pass boundaries by `hook`.
-    auto transform_group = [&](int grp, auto& hook) __attribute__((always_inline)) {
+    auto transform_group = [&](int grp, auto& hook, auto direct) __attribute__((always_inline)) {
         C* const gd = dst0 + group_base(grp);
+        constexpr bool FR = decltype(direct)::value;
 
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
-            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0>(
-                lds, lds2, tsv, ta, tb, tid, w, hook);
+            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0, FR>(
+                lds, lds2, tsv, ta, tb, tid, w, hook, pre);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
             if constexpr (STAGED) {
@@ -629,8 +678,8 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         } else {
             // inverse: M-point DFTs first (in LDS), then VEC-point DFTs with
             // the conjugate inter-stage twiddle; the last pass stores to HBM
-            C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP, 0>(
-                lds, lds2, tsm, ta, tb, tid, w, hook);
+            C* cur = fast_passes_lds<T, LM, 0, NPM, LVEC, LN, LE, NT, false, false, false, FULLTAB, MF, OOP, 0, FR>(
+                lds, lds2, tsm, ta, tb, tid, w, hook, pre);
             const T sc = MF ? scale * inv_s0 : scale;
             if constexpr (STAGED) {
                 cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, false, true, FULLTAB, MF, OOP, NP1>(
@@ -668,14 +717,23 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
 
     if constexpr (LAPX) {
         // zm_laplace_forward's x pass: the workgroup walks a contiguous chunk
-        // of global rows and transforms each row's Laplacian.  The rows above
-        // and below come from registers (this thread's elements of the
-        // previous row and of the prefetched next row), the x neighbours from
-        // the row staged in LDS, so the Laplacian costs no HBM traffic beyond
-        // the two rows bordering the chunk.  Rows run two ahead: while row gr
-        // is transformed, row gr + 2's loads are in flight into the register
-        // set row gr came from.
-        static_assert(FWD && LLPI == 0 && !MF, "the fused Laplacian runs ahead of one-line forward x passes");
+        // of global rows and transforms each row's Laplacian.  Of every row,
+        // thread tid holds the EPT elements (m, v) with v = tid mod VEC and
+        // m = (tid div VEC) EPT + k, k < EPT (element m VEC + v of the line):
+        // loads stay coalesced along v, and the x neighbours m -+ 1 of its
+        // elements are its own but at k = 0 and k = EPT - 1.  Those two come
+        // from the neighbouring thread through a small exchange area in LDS,
+        // written one row ahead and double-buffered.  The rows above and
+        // below are this thread's elements of the previous and the next row,
+        // in registers; rows run two ahead (while row gr is transformed, row
+        // gr + 2's loads are in flight into the register set row gr came
+        // from).  Per row this costs the plain pass's staging writes, two
+        // exchange writes and reads, and no barrier beyond the plain pass's.
+        static_assert(FWD && LLPI == 0 && !MF && FULLTAB, "the fused Laplacian runs ahead of one-line forward x passes");
+        static_assert(M % EPT == 0 && NT == (M / EPT) * VEC, "thread layout of the rows");
+        constexpr int NG = M / EPT;  // groups of EPT consecutive m
+        // exchange area after the PRE table: [buffer][end][thread]
+        C* const xb = tabs + TP + N;
         C rowa[EPT], rowb[EPT], up[EPT];
         const int chunk = (groups + (int)gridDim.x - 1) / (int)gridDim.x;
         const int r0 = (int)blockIdx.x * chunk;
@@ -688,14 +746,19 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             const int lr = ((gr >> (g.lb + lp)) << g.lb) + (gr & (g.b - 1));
             return (rid << ll) + lr;
         };
+        auto elem = [&](int k) -> int { return (((tid >> LVEC) * EPT + k) << LVEC) + (tid & (VEC - 1)); };
         auto fetch_row = [&](C* dst, int gr, auto k0c, auto k1c) __attribute__((always_inline)) {
             constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
             const C* gb = src0 + group_base(line_of(gr));
 #pragma unroll
             for (int k = K0; k < K1; k++) {
-                const int rem = (tid + k * NT) & (N - 1);
-                dst[k] = ldg(gb + off(rem >> ll, 0, rem & (L - 1)));
+                const int e = elem(k);
+                dst[k] = ldg(gb + off(e >> ll, 0, e & (L - 1)));
             }
+        };
+        auto put_ends = [&](const C* x, int buf) {
+            xb[(buf * 2 + 0) * NT + tid] = x[0];
+            xb[(buf * 2 + 1) * NT + tid] = x[EPT - 1];
         };
         using KB = std::integral_constant<int, 0>;
         using KE = std::integral_constant<int, EPT>;
@@ -703,28 +766,31 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             fetch_row(up, r0 - 1, KB{}, KE{});
             fetch_row(rowa, r0, KB{}, KE{});
             fetch_row(rowb, r0 + 1, KB{}, KE{});
+            put_ends(rowa, 0);
         }
-        __syncthreads();  // tables
-        auto row_step = [&](int gr, C* x, C* y) __attribute__((always_inline)) {
+        __syncthreads();  // tables, the first row's ends
+        // x neighbours across the threads' ends: m = 0 / M - 1 wrap to the
+        // neighbouring lane's other end, the reference's lane shift at the
+        // global x edge (laplace.cu:148, 160)
+        const int v = tid & (VEC - 1), gq = tid >> LVEC;
+        const int lt = gq > 0 ? tid - VEC : (NG - 1) * VEC + ((v - 1) & (VEC - 1));
+        const int rt = gq < NG - 1 ? tid + VEC : ((v + 1) & (VEC - 1));
+        auto row_step = [&](int gr, C* x, C* y, auto bufc) __attribute__((always_inline)) {
+            constexpr int BUF = decltype(bufc)::value;
             asm volatile("" : "+v"(tid));  // per-row addresses, not hoisted (registers)
-            // stage row gr, then its Laplacian over it (rows gr -+ 1 from
-            // registers); `up` becomes row gr for the next row
-#pragma unroll
-            for (int k = 0; k < EPT; k++) lds[fidx<T, LN>(tid + k * NT)] = x[k];
-            __syncthreads();
-            C lap[EPT];
+            const C lft = xb[(BUF * 2 + 1) * NT + lt], rgt = xb[(BUF * 2 + 0) * NT + rt];
 #pragma unroll
             for (int k = 0; k < EPT; k++) {
-                lap[k] = row_laplacian<T, LVEC, LM, LN>(lds, tid + k * NT, up[k], y[k], x[k], scale, apply_scale != 0);
-                up[k] = x[k];
+                const C l = k > 0 ? x[k - 1] : lft, r = k < EPT - 1 ? x[k + 1] : rgt;
+                lds[fidx<T, LN>(elem(k))] = lap5<T>(up[k], y[k], x[k], l, r, scale, apply_scale != 0);
             }
-            __syncthreads();
 #pragma unroll
-            for (int k = 0; k < EPT; k++) lds[fidx<T, LN>(tid + k * NT)] = lap[k];
+            for (int k = 0; k < EPT; k++) up[k] = x[k];
             __syncthreads();
             const bool more = gr + 2 <= r1;  // row r1 is the chunk's last "below"
             auto hook = [&](auto hc) {
                 constexpr int h = decltype(hc)::value;
+                if constexpr (h == 0) put_ends(y, BUF ^ 1);  // the next row's ends
                 if constexpr (h < HN) {
                     constexpr int K0 = h * EPT / HN, K1 = (h + 1) * EPT / HN;
                     if constexpr (K1 > K0) {
@@ -733,12 +799,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
                 }
             };
             hook(std::integral_constant<int, 0>{});
-            transform_group(line_of(gr), hook);
-            __syncthreads();  // the next row's staging overwrites LDS
+            transform_group(line_of(gr), hook, std::false_type{});
+            __syncthreads();  // the next row's writes overwrite LDS
         };
         for (int gr = r0; gr < r1; gr += 2) {
-            row_step(gr, rowa, rowb);
-            if (gr + 1 < r1) row_step(gr + 1, rowb, rowa);
+            row_step(gr, rowa, rowb, std::integral_constant<int, 0>{});
+            if (gr + 1 < r1) row_step(gr + 1, rowb, rowa, std::integral_constant<int, 1>{});
         }
         return;
     }
@@ -776,8 +842,10 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
             s0 = ldexpf(1.0f, MFMA_SCALE_LOG2 - ex);
             inv_s0 = ldexpf(1.0f, ex - MFMA_SCALE_LOG2);
         }
-        stage();
-        __syncthreads();
+        if constexpr (!DIRECT) {
+            stage();
+            __syncthreads();
+        }
         const int nxt = grp + gridDim.x;
         // The next group's loads are spread over the pass boundaries (HN
         // points: after staging and after every LDS pass before the last); a
@@ -792,8 +860,10 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         auto hook = [&](auto hc) {
             constexpr int h = decltype(hc)::value;
             if constexpr (SPREAD && h < HN) {
-                constexpr int K0 = h * EPT / HN, K1 = (h + 1) * EPT / HN;
-                if constexpr (K1 > K0) {
+                // DIRECT: the first pass reads pre[], so the first slice
+                // waits for it to end (point 1 issues slices 0 and 1)
+                constexpr int K0 = (DIRECT && h == 1 ? 0 : h) * EPT / HN, K1 = (h + 1) * EPT / HN;
+                if constexpr (K1 > K0 && !(DIRECT && h == 0)) {
                     if (nxt < groups)
                         fetch_part(std::integral_constant<int, K0>{}, std::integral_constant<int, K1>{}, nxt);
                 }
@@ -802,7 +872,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         if constexpr (SPREAD) hook(std::integral_constant<int, 0>{});
         else if (nxt < groups) fetch(nxt);
 
-        transform_group(grp, hook);
+        transform_group(grp, hook, std::bool_constant<DIRECT>{});
         if constexpr (MF) {
             if (nxt < groups) group_max((it + 1) & 1);
         }
@@ -823,9 +893,11 @@ int lines_per_iter_log2(int n) {
 }
 
 // LDS bytes of the specialised kernel for element size `es`
-size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false) {
+size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false, bool lapx = false) {
     const int n = vec * m;
     const int e = n << lines_per_iter_log2(n);
+    // lapx: the exchange area of the rows' thread ends, 2 buffers x 2 ends
+    if (lapx) return fast_lds(es, vec, m, fulltab, mf) + es * 4 * (e / EPT);
     // the two per-pass tables, then W_N (or the per-thread PRE table of the
     // same size) or the two-level pair
     const int tab = vec + m + (fulltab ? n : 64 + (n / 64 > 0 ? n / 64 : 1));
@@ -850,7 +922,7 @@ template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool M
 static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a,
                                    const Cx<T>* tables, int batch, double scale, int grid_x, hipStream_t stream) {
     constexpr int LN = LVEC + LM, N = 1 << LN, E = N << LLPI;
-    const size_t lds = fast_lds(sizeof(Cx<T>), 1 << LVEC, 1 << LM, FULLTAB, MF);
+    const size_t lds = fast_lds(sizeof(Cx<T>), 1 << LVEC, 1 << LM, FULLTAB, MF, LAPX);
     const void* fn = fast_fn<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPX>();
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -925,7 +997,7 @@ struct OccOp {
     template <int V, int MM, int LL, bool F, bool FT, bool MF, bool LX>
     int run() {
         const void* fn = fast_fn<T, V, MM, LL, F, FT, MF, LX>();
-        const size_t lds = fast_lds(sizeof(Cx<T>), vec, m, FT, MF);
+        const size_t lds = fast_lds(sizeof(Cx<T>), vec, m, FT, MF, LX);
         const int nthreads = ((1 << (V + MM)) << LL) / EPT;
         if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int blocks = 0;
@@ -955,7 +1027,8 @@ int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapx) {
 
 template <typename T>
 bool fast_line_lapx(int vec, int m, bool mf) {
-    return fast_line_supported<T>(vec, m) && (mf ? lapx_built<T, true>(vec, m) : lapx_built<T, false>(vec, m));
+    return fast_line_supported<T>(vec, m) && (mf ? lapx_built<T, true>(vec, m) : lapx_built<T, false>(vec, m)) &&
+           fast_line_fulltab<T>(vec, m, mf) && fast_lds(sizeof(Cx<T>), vec, m, true, mf, true) <= kLdsLimit;
 }
 
 template <typename T>
