@@ -590,8 +590,15 @@ def main():
         "stages": stage_roofline(stages, field * esize),
         "step_bytes_2sweep": bytes_2sweep,
         "step_bytes_textbook": bytes_textbook,
+        # model byte counts over the step time: with the Laplacian fused into
+        # the x pass the kernels move fewer bytes than the 2-sweep model
+        # counts, so that ratio can pass 1 and is not a bandwidth; the
+        # bandwidth is step_hbm_frac_moved (the bytes the kernels move)
         "step_hbm_frac_2sweep": round(step_frac, 4),
         "step_hbm_frac_textbook": round(bytes_textbook / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "step_bytes_moved": extra["step_chain"]["kernel_sweeps_per_step"] * sweep,
+        "step_hbm_frac_moved": round(extra["step_chain"]["kernel_sweeps_per_step"] * sweep / (ms * 1e-3) / 1e9
+                                     / HBM_PEAK_GBS, 4),
     })
     if plan.shape.b >= 2:
         extra["derivatives"] = time_derivatives(plan, x, n, 10, torch)
