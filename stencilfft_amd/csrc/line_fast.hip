@@ -736,7 +736,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         C* const xb = tabs + TP + N;
         C rowa[EPT], rowb[EPT], up[EPT];
         const int chunk = (groups + (int)gridDim.x - 1) / (int)gridDim.x;
-        const int r0 = (int)blockIdx.x * chunk;
+        // short chunks (two border rows re-read per chunk) run in XCD-aware
+        // order, so that a chunk's border rows were just read into the same
+        // L2 by its neighbours: 4096^2 (8 rows per chunk) 0.089 -> 0.073 ms;
+        // at 8192^2 (32 rows) that order took 0.232 -> 0.261 ms
+        const int wg = chunk < 16 ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+        const int r0 = wg * chunk;
         const int r1 = r0 + chunk < groups ? r0 + chunk : groups;
         // global row gr -> its line: exchange group o = rid, local row lr
         // (zmodel/utils.h:176-185)
