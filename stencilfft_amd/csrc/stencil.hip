@@ -540,12 +540,12 @@ __global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict_
 // cid -+ 1) are near it in XCD-aware block order, so those rows were just
 // fetched into the same L2.  Against the row strip kernel, which re-reads the rows of
 // planes gd +- 1 from L2 (FETCH 1.32x the field at 512^3), this took the
-// 512^3 FP32 Laplacian from 0.60 to 0.41 ms; in FP64 the strip kernel stayed
-// ahead (0.93 against 0.95 ms), so FP64 keeps it where it applies.
-// KE = R L / 256 tile elements per thread; KY, HX: y-halo and x-halo elements
-// per thread (guarded).
-template <typename E, typename S, int R, int KE, int KY, int HX>
-__global__ void __launch_bounds__(256) laplace_zmarch_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, int tz,
+// 512^3 FP32 Laplacian from 0.60 to 0.41 ms (0.40 with 512 threads), and
+// the FP64 one from 0.93 to 0.90 ms with 512 threads.
+// NTH threads per workgroup; KE = R L / NTH tile elements per thread; KY, HX:
+// y-halo and x-halo elements per thread (guarded).
+template <typename E, typename S, int R, int KE, int KY, int HX, int NTH = 256>
+__global__ void __launch_bounds__(NTH) laplace_zmarch_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g, int tz,
                                                    int ltiles, S scale, int apply_scale) {
 #pragma clang fp contract(off)
     using O = EOps<E>;
@@ -582,14 +582,14 @@ __global__ void __launch_bounds__(256) laplace_zmarch_kernel(const E* __restrict
     auto load_plane = [&](int z, E* c) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < KE; k++) {
-            const int e = tid + 256 * k;
+            const int e = tid + NTH * k;
             c[k] = src[row_off(z, r0 + (e >> g.ll), cid) + (e & (L - 1))];
         }
     };
     auto load_halo = [&](int z, E* yh, E* xh) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < KY; k++) {
-            const int e = tid + 256 * k;
+            const int e = tid + NTH * k;
             if (e < nyh) {
                 const int side = e >> g.ll;  // 0: row r0 - 1, 1: row r0 + R
                 yh[k] = src[row_off(z, side ? r0 + R : r0 - 1, cid) + (e & (L - 1))];
@@ -597,7 +597,7 @@ __global__ void __launch_bounds__(256) laplace_zmarch_kernel(const E* __restrict
         }
 #pragma unroll
         for (int k = 0; k < HX; k++) {
-            const int e = tid + 256 * k;
+            const int e = tid + NTH * k;
             if (e < nxh) {
                 const int row = e >> (g.lvec + 1), side = (e >> g.lvec) & 1, jj = e & (VEC - 1);
                 if (side == 0) {
@@ -625,17 +625,17 @@ __global__ void __launch_bounds__(256) laplace_zmarch_kernel(const E* __restrict
         }
 #pragma unroll
         for (int k = 0; k < KE; k++) {
-            const int e = tid + 256 * k;
+            const int e = tid + NTH * k;
             tile[((e >> g.ll) + 1) * W + VEC + (e & (L - 1))] = pl[C0][k];
         }
 #pragma unroll
         for (int k = 0; k < KY; k++) {
-            const int e = tid + 256 * k;
+            const int e = tid + NTH * k;
             if (e < nyh) tile[(e >> g.ll) * (R + 1) * W + VEC + (e & (L - 1))] = yh[H][k];
         }
 #pragma unroll
         for (int k = 0; k < HX; k++) {
-            const int e = tid + 256 * k;
+            const int e = tid + NTH * k;
             if (e < nxh) {
                 const int row = e >> (g.lvec + 1), side = (e >> g.lvec) & 1, jj = e & (VEC - 1);
                 tile[(row + 1) * W + (side ? VEC + L : 0) + jj] = xh[H][k];
@@ -644,7 +644,7 @@ __global__ void __launch_bounds__(256) laplace_zmarch_kernel(const E* __restrict
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < KE; k++) {
-            const int e = tid + 256 * k;
+            const int e = tid + NTH * k;
             const int r = (e >> g.ll) + 1, c = e & (L - 1);
             const int pos = r * W + VEC + c;
             const E a = tile[pos - W], b = tile[pos + W], l = tile[pos - VEC], rr = tile[pos + VEC];
@@ -733,8 +733,9 @@ static bool zmarch_ok(const Geo& g, size_t es) {
     return R * g.vec <= 256 && g.n % R == 0 && (size_t)(R + 2) * (size_t)(g.l + 2 * g.vec) * es <= 65536;
 }
 
-template <typename E, typename S, int R, int KE, int KY>
+template <typename E, typename S, int R, int KE, int KY, int NTH = 256>
 static hipError_t laplace_zmarch_inst(const E* in, E* out, const Geo& g, int batch, double coef, hipStream_t stream) {
+    if (KE * NTH != R * g.l || KY * NTH < 2 * g.l || 2 * R * g.vec > 2 * NTH) return hipErrorInvalidValue;
     const int ntiles = g.n / R;
     int ltiles = 0;
     while ((1 << ltiles) < ntiles) ltiles++;
@@ -745,13 +746,13 @@ static hipError_t laplace_zmarch_inst(const E* in, E* out, const Geo& g, int bat
     while (tz > 32 && (int64_t)ntiles * g.p * (g.n / tz) * batch < 512) tz >>= 1;
     const unsigned blocks = (unsigned)((int64_t)ntiles * g.p * (g.n / tz));
     const size_t lds = sizeof(E) * (size_t)(R + 2) * (size_t)(g.l + 2 * g.vec);
-    const bool two = 2 * R * g.vec > 256;
+    const bool two = 2 * R * g.vec > NTH;
     if (two)
-        hipLaunchKernelGGL((laplace_zmarch_kernel<E, S, R, KE, KY, 2>), dim3(blocks, (unsigned)batch), dim3(256), lds,
-                           stream, in, out, g, tz, ltiles, (S)coef, coef != 1.0 ? 1 : 0);
+        hipLaunchKernelGGL((laplace_zmarch_kernel<E, S, R, KE, KY, 2, NTH>), dim3(blocks, (unsigned)batch), dim3(NTH),
+                           lds, stream, in, out, g, tz, ltiles, (S)coef, coef != 1.0 ? 1 : 0);
     else
-        hipLaunchKernelGGL((laplace_zmarch_kernel<E, S, R, KE, KY, 1>), dim3(blocks, (unsigned)batch), dim3(256), lds,
-                           stream, in, out, g, tz, ltiles, (S)coef, coef != 1.0 ? 1 : 0);
+        hipLaunchKernelGGL((laplace_zmarch_kernel<E, S, R, KE, KY, 1, NTH>), dim3(blocks, (unsigned)batch), dim3(NTH),
+                           lds, stream, in, out, g, tz, ltiles, (S)coef, coef != 1.0 ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -761,7 +762,8 @@ static hipError_t laplace_zmarch(const E* in, E* out, const Geo& g, int batch, d
         case 32: return laplace_zmarch_inst<E, S, 8, 1, 1>(in, out, g, batch, coef, stream);
         case 64: return laplace_zmarch_inst<E, S, 8, 2, 1>(in, out, g, batch, coef, stream);
         case 128: return laplace_zmarch_inst<E, S, 8, 4, 1>(in, out, g, batch, coef, stream);
-        case 256: return laplace_zmarch_inst<E, S, 8, 8, 2>(in, out, g, batch, coef, stream);
+        // 512 threads: 0.405 -> 0.399 ms (FP32), 0.95 -> 0.90 ms (FP64) at 512^3
+        case 256: return laplace_zmarch_inst<E, S, 8, 4, 1, 512>(in, out, g, batch, coef, stream);
         case 512: return laplace_zmarch_inst<E, S, 4, 8, 4>(in, out, g, batch, coef, stream);
         case 1024: return laplace_zmarch_inst<E, S, 2, 8, 8>(in, out, g, batch, coef, stream);
         default: return hipErrorInvalidValue;
@@ -775,10 +777,11 @@ static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int 
     const dim3 grid((unsigned)((g.field + threads - 1) / threads), (unsigned)batch);
     const E* src = static_cast<const E*>(in);
     E* dst = static_cast<E*>(out);
-    // 3D: the depth march, except FP64 where the strip kernel applies (there
-    // it measured faster)
+    // 3D: the depth march, except FP64 with L != 256 where the strip kernel
+    // applies (the 256-thread march measured slower there; L = 256 runs the
+    // 512-thread march, 0.90 against 0.93 ms for the strip kernel at 512^3)
     constexpr bool fp64 = sizeof(S) == 8;
-    if (op == OP_LAPLACE && zmarch_ok(g, sizeof(E)) && !(fp64 && strip_ok(g, batch))) {
+    if (op == OP_LAPLACE && zmarch_ok(g, sizeof(E)) && !(fp64 && g.l != 256 && strip_ok(g, batch))) {
         return laplace_zmarch<E, S>(src, dst, g, batch, coef, stream);
     } else if (op == OP_LAPLACE && strip_ok(g, batch)) {
         return laplace_strip<E, S>(src, dst, g, batch, coef, stream);

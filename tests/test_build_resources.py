@@ -72,7 +72,8 @@ def kernels(tmp_path_factory):
 
 def test_every_object_targets_gfx950_and_holds_the_hot_kernels(kernels):
     fams = {k.split("<")[0] for k in kernels}
-    for f in ("zm::line_fast_kernel", "zm::repack_tiled_kernel", "zm::laplace_strip_kernel", "zm::deriv4_march_kernel"):
+    for f in ("zm::line_fast_kernel", "zm::repack_tiled_kernel", "zm::laplace_strip_kernel", "zm::deriv4_march_kernel",
+              "zm::laplace_zmarch_kernel"):
         assert f in fams, f
 
 
