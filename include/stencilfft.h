@@ -134,11 +134,11 @@ zm_status zm_forward(zm_plan* plan, const void* in, void* out, void* scratch, vo
 /* Forward transform of `scale` times the periodic 5-point (2D) / 7-point (3D)
  * Laplacian of `in`: the reference's laplace followed by forward_fft
  * (zmodel/main.cu:50-63, laplace.cu:73-169 / laplace3d.cu:90-225, fft.cu:
- * 199-339), bit for bit zm_forward(zm_laplace(in)) with `scale`.  On 2D FP32
- * plans with the specialised path and N >= 4096 the Laplacian is formed inside
- * the x pass (three sweeps of the field instead of four); elsewhere it is the
- * two calls.  ZM_PATH_TWO_STAGE plans get ZM_ERR_UNSUPPORTED.  Buffers as
- * zm_forward. */
+ * 199-339), bit for bit zm_forward(zm_laplace(in)) with `scale`.  On FP32
+ * plans with the specialised path (2D: N >= 4096; 3D: lines of N <= 1024
+ * points whose row groups lie in one B-block) the Laplacian is formed inside
+ * the x pass, one sweep of the field less; elsewhere it is the two calls.
+ * ZM_PATH_TWO_STAGE plans get ZM_ERR_UNSUPPORTED.  Buffers as zm_forward. */
 zm_status zm_laplace_forward(zm_plan* plan, const void* in, void* out, double scale, void* scratch, void* stream);
 
 /* Scratch bytes available to zm_forward_noalloc on each GPU. */

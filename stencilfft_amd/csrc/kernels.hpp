@@ -53,11 +53,16 @@ bool fast_line_supported(int vec, int m);
 template <typename T>
 int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapx);
 // whether the fused Laplacian + x pass (lapx) exists for this (VEC, M) pair
+// in `dims` dimensions with block size `b`
 template <typename T>
-bool fast_line_lapx(int vec, int m, bool mf);
+bool fast_line_lapx(int vec, int m, bool mf, int dims, int b);
+// log2 of the lines per group of the specialised pass for N = vec * m
+int fast_line_lapx_lines_log2(int vec, int m);
 // mf: radix-16 passes as split-precision FP16 MFMA GEMMs (FP32 only).
-// lapx (2D forward x pass, fast_line_lapx): transform `scale` times the
-// 5-point Laplacian of the input (the fused Laplacian + forward transform).
+// lapx (forward x pass, fast_line_lapx): transform `scale` times the 5-point
+// (2D) / 7-point (3D) Laplacian of the input (the fused Laplacian + forward
+// transform).  3D: grid_x = (N / lines per group) row tiles times the number
+// of plane chunks, which must divide N.
 template <typename T>
 hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
                             int batch, bool fwd, double scale, int grid_x, bool mf, bool lapx, hipStream_t stream);

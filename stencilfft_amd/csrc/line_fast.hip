@@ -445,7 +445,7 @@ constexpr bool kDirectFirstPass = !MF && !FWD && LLPI == 0;
 // LAPX (2D forward x pass, one-line groups): transform scale * the 5-point
 // Laplacian of the input instead of the input (zm_laplace_forward).
 template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPX = false>
-__global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
+__global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI > 0) ? 2 : 1)
     line_fast_kernel(const Cx<T>* in, Cx<T>* out, Geo g, LineArgs a, const Cx<T>* __restrict__ tables, T scale,
                      int apply_scale, int groups) {
     using C = Cx<T>;
@@ -715,7 +715,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         }
     };
 
-    if constexpr (LAPX) {
+    if constexpr (LAPX && LLPI == 0) {
         // zm_laplace_forward's x pass: the workgroup walks a contiguous chunk
         // of global rows and transforms each row's Laplacian.  Of every row,
         // thread tid holds the EPT elements (m, v) with v = tid mod VEC and
@@ -820,6 +820,150 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT)
         return;
     }
 
+    if constexpr (LAPX) {
+        // zm_laplace_forward's x pass in 3D: a group is LPI consecutive global
+        // rows of one global plane, so the workgroup owns one tile of LPI rows
+        // (the x lines of those rows) and marches along global planes z0 ..
+        // z1 - 1, transforming each plane's tile of the 7-point Laplacian.
+        // Of every tile, thread tid holds rows l < LPI at line elements
+        // e = tid + NT k2, k2 < KP: the rows above and below (y -+ 1) are its
+        // own registers, the two rows bordering the tile come as halo
+        // elements, and the planes before and after (z -+ 1) are registers
+        // too (the previous and the prefetched next plane).  The x neighbours
+        // come from the raw tile staged in a second LDS buffer, written at the
+        // previous plane's first hook point, so they cost no barrier.  The
+        // arithmetic is the FP32 stencil's: bit for bit zm_laplace.
+        static_assert(FWD && !MF && FULLTAB, "the fused Laplacian runs ahead of forward x passes");
+        constexpr int LPI = 1 << LLPI, KP = EPT / LPI;
+        static_assert(KP * NT == N && KP >= 1, "tile layout");
+        C* const xr = tabs + TP + N;  // the raw tile of the current plane
+        const int lpg = a.lines_per_group;
+        const int ntile = g.n >> LLPI;
+        const int wg = (int)xcd_block(blockIdx.x, gridDim.x);
+        const int tile = wg % ntile, zc = wg / ntile;
+        const int tz = g.n / ((int)gridDim.x / ntile);
+        const int z0 = zc * tz, z1 = z0 + tz;
+        const int gy0 = tile << LLPI;
+        // global index -> (rank coordinate, local index) (zmodel/utils.h:176-185)
+        auto split = [&](int gi, int& r, int& li) {
+            gi &= g.n - 1;
+            r = (gi >> g.lb) & (g.p - 1);
+            li = ((gi >> (g.lb + lp)) << g.lb) + (gi & (g.b - 1));
+        };
+        int rid0, lr0, ridu, lru, ridd, lrd;
+        split(gy0, rid0, lr0);
+        split(gy0 - 1, ridu, lru);
+        split(gy0 + LPI, ridd, lrd);
+        // line group of the tile at plane gz, and the row base of a halo row
+        auto group_of = [&](int gz) -> int {
+            int did, ldp;
+            split(gz, did, ldp);
+            return (int)((((int64_t)((did << lp) + rid0)) * lpg + ldp * L + lr0) >> LLPI);
+        };
+        auto row_base = [&](int gz, int rid, int lr) -> int64_t {
+            int did, ldp;
+            split(gz, did, ldp);
+            return (int64_t)(((did << lp) + rid) << lp) * g.local + ((int64_t)(ldp * L + lr) << ll);
+        };
+        // every per-thread address is recomputed per plane (the empty asm on
+        // tid in plane_step), which keeps the pass at two waves per SIMD
+        auto elem = [&](int k2) -> int { return tid + NT * k2; };
+        auto fetch_plane = [&](C* dst, int gz, auto k0c, auto k1c) __attribute__((always_inline)) {
+            constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
+            const C* gb = src0 + group_base(group_of(gz));
+#pragma unroll
+            for (int k = K0; k < K1; k++) {
+                const int l = k / KP, e = elem(k % KP);
+                dst[k] = ldg(gb + off(e >> ll, l, e & (L - 1)));
+            }
+        };
+        C up[EPT], rowa[EPT], rowb[EPT], hal[2][2][KP];
+        auto fetch_halo = [&](C (*h)[KP], int gz) {
+            const C* bu = src0 + row_base(gz, ridu, lru);
+            const C* bd = src0 + row_base(gz, ridd, lrd);
+#pragma unroll
+            for (int k2 = 0; k2 < KP; k2++) {
+                const int e = elem(k2);
+                h[0][k2] = ldg(bu + off(e >> ll, 0, e & (L - 1)));
+                h[1][k2] = ldg(bd + off(e >> ll, 0, e & (L - 1)));
+            }
+        };
+        auto stage_raw = [&](const C* x) {
+#pragma unroll
+            for (int k = 0; k < EPT; k++) xr[fidx<T, LN>(((k / KP) << LN) + elem(k % KP))] = x[k];
+        };
+        // x neighbours of line element e (packed column pos = j VEC + jj of
+        // member rank gm): j -+ 1, across the rank boundary at j = 0 / B - 1,
+        // and at the global x edge the other end's lane jj -+ 1 (the
+        // reference's lane shift, laplace3d.cu:90-225)
+        auto x_lo = [&](int e) -> int {
+            const int gm = e >> ll, pos = e & (L - 1), j = pos >> LVEC, jj = pos & (VEC - 1);
+            const int lastj = (g.b - 1) << LVEC;
+            return j > 0 ? e - VEC
+                         : gm > 0 ? ((gm - 1) << ll) + lastj + jj : ((g.p - 1) << ll) + lastj + ((jj - 1) & (VEC - 1));
+        };
+        auto x_hi = [&](int e) -> int {
+            const int gm = e >> ll, pos = e & (L - 1), j = pos >> LVEC, jj = pos & (VEC - 1);
+            return j < g.b - 1 ? e + VEC : gm < g.p - 1 ? ((gm + 1) << ll) + jj : ((jj + 1) & (VEC - 1));
+        };
+        using KB = std::integral_constant<int, 0>;
+        using KE = std::integral_constant<int, EPT>;
+        if (z0 < z1) {
+            fetch_plane(up, z0 - 1, KB{}, KE{});
+            fetch_plane(rowa, z0, KB{}, KE{});
+            fetch_plane(rowb, z0 + 1, KB{}, KE{});
+            fetch_halo(hal[0], z0);
+            stage_raw(rowa);
+        }
+        __syncthreads();  // tables, the first plane's raw tile
+        auto plane_step = [&](int gz, C* x, C* y, auto bufc) __attribute__((always_inline)) {
+#pragma clang fp contract(off)
+            constexpr int BUF = decltype(bufc)::value;
+            asm volatile("" : "+v"(tid));  // the transform's addresses, per plane
+#pragma unroll
+            for (int k = 0; k < EPT; k++) {
+                const int l = k / KP, k2 = k % KP;
+                const C c = x[k];
+                const C ya = l > 0 ? x[k - KP] : hal[BUF][0][k2];
+                const C yb = l < LPI - 1 ? x[k + KP] : hal[BUF][1][k2];
+                const C xa = xr[fidx<T, LN>((l << LN) + x_lo(elem(k2)))];
+                const C xb = xr[fidx<T, LN>((l << LN) + x_hi(elem(k2)))];
+                const C za = up[k], zb = y[k];
+                const T sy = (ya.x - c.x) + (yb.x - c.x), ty = (ya.y - c.y) + (yb.y - c.y);
+                const T sx = (xa.x - c.x) + (xb.x - c.x), tx = (xa.y - c.y) + (xb.y - c.y);
+                const T sz = (za.x - c.x) + (zb.x - c.x), tzz = (za.y - c.y) + (zb.y - c.y);
+                C s = C{(sy + sx) + sz, (ty + tx) + tzz};
+                if (apply_scale) s = C{s.x * scale, s.y * scale};
+                lds[fidx<T, LN>((l << LN) + elem(k2))] = s;
+            }
+#pragma unroll
+            for (int k = 0; k < EPT; k++) up[k] = x[k];
+            __syncthreads();  // the Laplacian staged; the raw tile read
+            const bool more = gz + 2 <= z1;  // plane z1 is the chunk's last "after"
+            auto hook = [&](auto hc) {
+                constexpr int h = decltype(hc)::value;
+                if constexpr (h == 0) {
+                    stage_raw(y);  // the next plane's raw tile
+                    if (gz + 1 < z1) fetch_halo(hal[BUF ^ 1], gz + 1);
+                }
+                if constexpr (h < HN) {
+                    constexpr int K0 = h * EPT / HN, K1 = (h + 1) * EPT / HN;
+                    if constexpr (K1 > K0) {
+                        if (more) fetch_plane(x, gz + 2, std::integral_constant<int, K0>{}, std::integral_constant<int, K1>{});
+                    }
+                }
+            };
+            hook(std::integral_constant<int, 0>{});
+            transform_group(group_of(gz), hook, std::false_type{});
+            __syncthreads();  // the next plane's writes overwrite LDS
+        };
+        for (int gz = z0; gz < z1; gz += 2) {
+            plane_step(gz, rowa, rowb, std::integral_constant<int, 0>{});
+            if (gz + 1 < z1) plane_step(gz + 1, rowb, rowa, std::integral_constant<int, 1>{});
+        }
+        return;
+    }
+
     int grp = blockIdx.x;
     int it = 0;
     if (grp < groups) {
@@ -907,12 +1051,13 @@ int lines_per_iter_log2(int n) {
 size_t fast_lds(size_t es, int vec, int m, bool fulltab, bool mf = false, bool lapx = false) {
     const int n = vec * m;
     const int e = n << lines_per_iter_log2(n);
-    // lapx: the exchange area of the rows' thread ends, 2 buffers x 2 ends
-    if (lapx) return fast_lds(es, vec, m, fulltab, mf) + es * 4 * (e / EPT);
+    const int shift = n >= 4096 ? (es == 16 ? 6 : 4) : 5;
+    // lapx, one-line groups (2D): the exchange area of the rows' thread
+    // ends, 2 buffers x 2 ends; several lines (3D): the raw tile
+    if (lapx) return fast_lds(es, vec, m, fulltab, mf) + es * (e == n ? 4 * (e / EPT) : e + (e >> shift) + 1);
     // the two per-pass tables, then W_N (or the per-thread PRE table of the
     // same size) or the two-level pair
     const int tab = vec + m + (fulltab ? n : 64 + (n / 64 > 0 ? n / 64 : 1));
-    const int shift = n >= 4096 ? (es == 16 ? 6 : 4) : 5;
     const int padded = e + (e >> shift) + 1;
     return es * (padded * (mf ? 2 : 1) + tab) + (mf ? MFMA_W_BYTES + MFMA_MAX_BYTES : 0);
 }
@@ -953,15 +1098,19 @@ static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, co
 template <typename T, bool MF>
 constexpr bool kBuildTwoLevel = !(sizeof(T) == 4 && !MF);
 
-// The fused Laplacian + x pass (LAPX) is built for FP32 vector-ALU one-line
-// groups (N >= 4096); its register sets leave no room in FP64 or beside the
-// MFMA operands.
+// The fused Laplacian + x pass (LAPX) is built for the FP32 vector ALU: 2D
+// one-line groups (N >= 4096) and 3D groups of several lines of N <= 1024;
+// its register sets leave no room in FP64 or beside the MFMA operands.
 template <typename T, bool MF, int V, int MM>
-constexpr bool kBuildLapx = sizeof(T) == 4 && !MF && V + MM >= 12;
+constexpr bool kBuildLapx = sizeof(T) == 4 && !MF && (V + MM >= 12 || V + MM <= 10);
 
+// 2D: one-line groups; 3D: groups of LPI = 4096 / N lines that lie in one
+// B-block of rows (consecutive global rows)
 template <typename T, bool MF>
-bool lapx_built(int vec, int m) {
-    return sizeof(T) == 4 && !MF && vec * m >= 4096;
+bool lapx_built(int vec, int m, int dims, int b) {
+    if (!(sizeof(T) == 4 && !MF)) return false;
+    const int n = vec * m;
+    return dims == 2 ? n >= 4096 : (n <= 1024 && (4096 / n) <= b);
 }
 
 template <typename T, bool MF, typename Op>
@@ -1028,7 +1177,7 @@ bool fast_line_supported(int vec, int m) {
 
 template <typename T>
 int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapx) {
-    if (lapx && !(fwd && (mf ? lapx_built<T, true>(vec, m) : lapx_built<T, false>(vec, m)))) return 0;
+    if (lapx && !(fwd && !mf && sizeof(T) == 4)) return 0;
     if constexpr (sizeof(T) == 4) {
         if (mf) return with_fast<T, true>(vec, m, fwd, lapx, OccOp<T>{vec, m});
     }
@@ -1037,15 +1186,18 @@ int fast_line_occupancy(int vec, int m, bool fwd, bool mf, bool lapx) {
 }
 
 template <typename T>
-bool fast_line_lapx(int vec, int m, bool mf) {
-    return fast_line_supported<T>(vec, m) && (mf ? lapx_built<T, true>(vec, m) : lapx_built<T, false>(vec, m)) &&
+bool fast_line_lapx(int vec, int m, bool mf, int dims, int b) {
+    return fast_line_supported<T>(vec, m) &&
+           (mf ? lapx_built<T, true>(vec, m, dims, b) : lapx_built<T, false>(vec, m, dims, b)) &&
            fast_line_fulltab<T>(vec, m, mf) && fast_lds(sizeof(Cx<T>), vec, m, true, mf, true) <= kLdsLimit;
 }
+
+int fast_line_lapx_lines_log2(int vec, int m) { return lines_per_iter_log2(vec * m); }
 
 template <typename T>
 hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
                             int batch, bool fwd, double scale, int grid_x, bool mf, bool lapx, hipStream_t stream) {
-    if (lapx && !(fwd && g.dims == 2 && fast_line_lapx<T>(g.vec, g.m, mf))) return hipErrorInvalidValue;
+    if (lapx && !(fwd && fast_line_lapx<T>(g.vec, g.m, mf, g.dims, g.b))) return hipErrorInvalidValue;
     const LaunchOp<T> op{in, out, g, a, tables, batch, scale, grid_x, stream};
     if constexpr (sizeof(T) == 4) {
         if (mf) return with_fast<T, true>(g.vec, g.m, fwd, lapx, LaunchOp<T>(op));
@@ -1097,8 +1249,8 @@ template bool fast_line_supported<float>(int, int);
 template bool fast_line_supported<double>(int, int);
 template int fast_line_occupancy<float>(int, int, bool, bool, bool);
 template int fast_line_occupancy<double>(int, int, bool, bool, bool);
-template bool fast_line_lapx<float>(int, int, bool);
-template bool fast_line_lapx<double>(int, int, bool);
+template bool fast_line_lapx<float>(int, int, bool, int, int);
+template bool fast_line_lapx<double>(int, int, bool, int, int);
 template hipError_t launch_line_fast<float>(const float2*, float2*, const Geo&, const LineArgs&, const float2*, int,
                                             bool, double, int, bool, bool, hipStream_t);
 template hipError_t launch_line_fast<double>(const double2*, double2*, const Geo&, const LineArgs&, const double2*,

@@ -297,10 +297,9 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
                                 : zm::fast_line_occupancy<float>(d.vec, d.m, true, mfma, false);
             const int oi = fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, false, false, false)
                                 : zm::fast_line_occupancy<float>(d.vec, d.m, false, mfma, false);
-            // the fused Laplacian + x pass: 2D only (it needs the rows above
-            // and below, not the planes before and after)
-            const bool lx = shape->dims == 2 && (fp64 ? zm::fast_line_lapx<double>(d.vec, d.m, false)
-                                                      : zm::fast_line_lapx<float>(d.vec, d.m, mfma));
+            // the fused Laplacian + x pass (FP32 vector ALU)
+            const bool lx = fp64 ? zm::fast_line_lapx<double>(d.vec, d.m, false, shape->dims, shape->b)
+                                 : zm::fast_line_lapx<float>(d.vec, d.m, mfma, shape->dims, shape->b);
             const int ol = !lx ? 0 : fp64 ? zm::fast_line_occupancy<double>(d.vec, d.m, true, false, true)
                                           : zm::fast_line_occupancy<float>(d.vec, d.m, true, mfma, true);
             if (e == hipSuccess && of > 0 && oi > 0) {
@@ -308,6 +307,14 @@ zm_status zm_plan_create(zm_plan** plan, const zm_shape* shape, zm_precision pre
                 p->fast_grid_fwd = of * cus;
                 p->fast_grid_inv = oi * cus;
                 p->fast_grid_lapx = ol * cus;
+                if (shape->dims == 3 && ol > 0) {
+                    // 3D: row tiles times plane chunks, about the resident
+                    // workgroups, the chunks at least two planes long
+                    const int ntile = n >> zm::fast_line_lapx_lines_log2(d.vec, d.m);
+                    int nzc = 1;
+                    while (2 * nzc * ntile <= ol * cus && 2 * nzc <= n / 2) nzc *= 2;
+                    p->fast_grid_lapx = ntile * nzc;
+                }
             }
         }
     }
@@ -480,17 +487,24 @@ struct Exec {
     }
 
     // The reference's laplace followed by forward_fft (zmodel/main.cu:50-63).
-    // With the fused pass the Laplacian is formed inside the x pass (three
-    // sweeps instead of four); otherwise the stencil writes `out` and the
+    // With the fused pass the Laplacian is formed inside the x pass (one
+    // sweep of the field less: 3 instead of 4 in 2D, 5 instead of 6 in 3D);
+    // otherwise the stencil writes `out` and the
     // forward chain runs in place on it (its first pass reads `out` whole
     // before the repack writes it).  Either way the result is bit for bit
     // zm_forward(zm_laplace(in)).
     zm_status laplace_forward(const void* in, void* out, void* s, double lap_scale) const {
         if (p->two_stage) return ZM_ERR_UNSUPPORTED;
         if (p->fast_grid_lapx > 0) {
+            const bool d3 = p->shape.dims == 3;
             ZM_TRY(line(in, s, 0, true, lap_scale, true));
-            ZM_TRY(repack(s, out, REPACK_2D, true));
-            return line(out, out, 1, true, 1.0);
+            ZM_TRY(repack(s, out, d3 ? REPACK_XY : REPACK_2D, true));
+            ZM_TRY(line(out, out, 1, true, 1.0));
+            if (d3) {
+                ZM_TRY(repack(out, s, REPACK_YZ, true));
+                ZM_TRY(line(s, out, 2, true, 1.0));
+            }
+            return ZM_OK;
         }
         ZM_TRY(launch_status(timed_launch(p, ZM_STAGE_LAPLACE, st, [&] {
             return zm::launch_stencil<T>(in, out, p->geo, p->batch, zm::OP_LAPLACE, true, lap_scale, false, st);

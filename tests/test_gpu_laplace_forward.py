@@ -1,10 +1,13 @@
 """zm_laplace_forward: the reference's Laplacian followed by forward_fft
 (zmodel/main.cu:50-63) as one call.
 
-On 2D FP32 plans with the specialised pass and N >= 4096 the Laplacian is
-formed inside the forward transform's x pass (rows above and below from
-registers, x neighbours from the row in LDS); elsewhere the call runs the
-stencil and then the transform.  Either way the result must be bit for bit
+On FP32 plans with the specialised pass the Laplacian is formed inside the
+forward transform's x pass: in 2D for N >= 4096 (rows above and below from
+registers, x neighbours from the thread itself and its neighbours' ends), in
+3D for lines of N <= 1024 points whose groups of rows lie in one B-block (a
+march along the planes: rows above and below and the planes before and after
+from registers, x neighbours from the raw tile in LDS).  Elsewhere the call
+runs the stencil and then the transform.  Either way the result must be bit for bit
 zm_forward(zm_laplace(x, scale)), whose own parity with the oracle
 (laplace.cu:73-169, laplace3d.cu:90-225, fft.cu:199-339) the other GPU tests
 hold.  Shapes cover both paths, the C2 shape on its smooth mode field, a
@@ -21,7 +24,8 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a
 
 # (dims, n, p, b, fused x pass expected in FP32)
 SHAPES = [(2, 256, 2, 8, False), (2, 1024, 2, 16, False), (2, 4096, 4, 16, True), (2, 4096, 2, 16, True),
-          (2, 8192, 4, 32, True), (2, 8192, 2, 32, True), (3, 256, 2, 8, False), (3, 512, 2, 16, False)]
+          (2, 8192, 4, 32, True), (2, 8192, 2, 32, True), (3, 256, 2, 8, False), (3, 512, 2, 16, True),
+          (3, 512, 4, 8, True), (3, 512, 2, 8, True), (3, 256, 4, 4, False)]
 
 
 def note(msg):
@@ -72,7 +76,7 @@ def test_laplace_forward_is_laplace_then_forward(dims, n, p, b, fused, precision
     expect_fused = fused and precision == "fp32"
     assert ("laplace" not in st) == expect_fused, st
     if expect_fused:
-        assert st["line_laplace"][0] == 1 and st["line"][0] == 1 and st["repack"][0] == 1
+        assert st["line_laplace"][0] == 1 and st["line"][0] == dims - 1 and st["repack"][0] == dims - 1, st
     del torch
 
 
