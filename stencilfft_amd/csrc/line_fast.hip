@@ -865,9 +865,14 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
             split(gz, did, ldp);
             return (int64_t)(((did << lp) + rid) << lp) * g.local + ((int64_t)(ldp * L + lr) << ll);
         };
-        // every per-thread address is recomputed per plane (the empty asm on
-        // tid in plane_step), which keeps the pass at two waves per SIMD
+        // the per-thread addresses are recomputed per plane (the empty asm on
+        // tid in plane_step), but for the x neighbours' line indices, kept
+        // from the start (elemh): 210 VGPRs, two waves per SIMD (with every
+        // address hoisted 260 VGPRs, one wave; with none 178 VGPRs and the
+        // 512^3 pass 0.570 against 0.497 ms)
+        const int tidh = threadIdx.x;
         auto elem = [&](int k2) -> int { return tid + NT * k2; };
+        auto elemh = [&](int k2) -> int { return tidh + NT * k2; };
         auto fetch_plane = [&](C* dst, int gz, auto k0c, auto k1c) __attribute__((always_inline)) {
             constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
             const C* gb = src0 + group_base(group_of(gz));
@@ -906,6 +911,12 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
             const int gm = e >> ll, pos = e & (L - 1), j = pos >> LVEC, jj = pos & (VEC - 1);
             return j < g.b - 1 ? e + VEC : gm < g.p - 1 ? ((gm + 1) << ll) + jj : ((jj + 1) & (VEC - 1));
         };
+        int xlo[KP], xhi[KP];
+#pragma unroll
+        for (int k2 = 0; k2 < KP; k2++) {
+            xlo[k2] = x_lo(elemh(k2));
+            xhi[k2] = x_hi(elemh(k2));
+        }
         using KB = std::integral_constant<int, 0>;
         using KE = std::integral_constant<int, EPT>;
         if (z0 < z1) {
@@ -926,8 +937,8 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
                 const C c = x[k];
                 const C ya = l > 0 ? x[k - KP] : hal[BUF][0][k2];
                 const C yb = l < LPI - 1 ? x[k + KP] : hal[BUF][1][k2];
-                const C xa = xr[fidx<T, LN>((l << LN) + x_lo(elem(k2)))];
-                const C xb = xr[fidx<T, LN>((l << LN) + x_hi(elem(k2)))];
+                const C xa = xr[fidx<T, LN>((l << LN) + xlo[k2])];
+                const C xb = xr[fidx<T, LN>((l << LN) + xhi[k2])];
                 const C za = up[k], zb = y[k];
                 const T sy = (ya.x - c.x) + (yb.x - c.x), ty = (ya.y - c.y) + (yb.y - c.y);
                 const T sx = (xa.x - c.x) + (xb.x - c.x), tx = (xa.y - c.y) + (xb.y - c.y);
