@@ -98,6 +98,21 @@ def test_c2_smooth_field_and_batch():
     assert _same_bits(pl.laplace_forward(xb, scale), pl.forward(pl.laplace(xb, scale)))
 
 
+def test_3d_batch_of_two():
+    """The 3D plane march on a batch of two fields (grid.y = batch): bit for
+    bit the two calls, field by field."""
+    import torch
+    from stencilfft_amd import Plan
+    n, p, b = 512, 4, 8
+    note(f"laplace_forward 3D N={n} P={p} B={b}, batch 2")
+    pl = Plan(3, n, p, b, precision="fp32", batch=2)
+    x = _field(3, n, 77, batch=2).to(torch.complex64)
+    scale = float(n * n)
+    got, st = _stages(pl, lambda: pl.laplace_forward(x, scale))
+    assert "line_laplace" in st, st
+    assert _same_bits(got, pl.forward(pl.laplace(x, scale)))
+
+
 def test_fallback_against_oracle(oracle_mod):
     import torch
     from stencilfft_amd import Plan
