@@ -442,8 +442,10 @@ constexpr bool kDirectFirstPass = !MF && !FWD && LLPI == 0;
 
 // Fused pass specialised for VEC = 2^LVEC, M = 2^LM, LPI = 2^LLPI lines per
 // iteration.  a.lpb is the number of iterations (line groups) per workgroup.
-// LAPX (2D forward x pass, one-line groups): transform scale * the 5-point
-// Laplacian of the input instead of the input (zm_laplace_forward).
+// LAPX (forward x pass): transform scale * the 5-point (2D, one-line groups)
+// or 7-point (3D, groups of several rows) Laplacian of the input instead of
+// the input (zm_laplace_forward).  The 3D form asks for two waves per SIMD
+// (at most 256 VGPRs), which its plane march fits without spills.
 template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPX = false>
 __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI > 0) ? 2 : 1)
     line_fast_kernel(const Cx<T>* in, Cx<T>* out, Geo g, LineArgs a, const Cx<T>* __restrict__ tables, T scale,
