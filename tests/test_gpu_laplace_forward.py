@@ -113,6 +113,22 @@ def test_3d_batch_of_two():
     assert _same_bits(got, pl.forward(pl.laplace(x, scale)))
 
 
+def test_3d_lines_of_1024_points():
+    """1024^3, P=2, B=32: groups of 4 rows (the only 3D shape class with
+    lines of 1024 points), FP32, bit for bit the two calls."""
+    import torch
+    from stencilfft_amd import Plan
+    n, p, b = 1024, 2, 32
+    note(f"laplace_forward 3D N={n} P={p} B={b}")
+    pl = Plan(3, n, p, b, precision="fp32")
+    x = _field(3, n, 78).to(torch.complex64)
+    scale = float(n * n)
+    got, st = _stages(pl, lambda: pl.laplace_forward(x, scale))
+    assert "line_laplace" in st, st
+    want = pl.forward(pl.laplace(x, scale))
+    assert _same_bits(got, want)
+
+
 def test_fallback_against_oracle(oracle_mod):
     import torch
     from stencilfft_amd import Plan
