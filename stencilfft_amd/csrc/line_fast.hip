@@ -1211,6 +1211,11 @@ template <typename T>
 hipError_t launch_line_fast(const Cx<T>* in, Cx<T>* out, const Geo& g, const LineArgs& a, const Cx<T>* tables,
                             int batch, bool fwd, double scale, int grid_x, bool mf, bool lapx, hipStream_t stream) {
     if (lapx && !(fwd && fast_line_lapx<T>(g.vec, g.m, mf, g.dims, g.b))) return hipErrorInvalidValue;
+    if (lapx && g.dims == 3) {
+        // the plane march: grid_x = row tiles x plane chunks, chunks dividing N
+        const int ntile = g.n >> lines_per_iter_log2(g.vec * g.m);
+        if (grid_x <= 0 || grid_x % ntile || g.n % (grid_x / ntile)) return hipErrorInvalidValue;
+    }
     const LaunchOp<T> op{in, out, g, a, tables, batch, scale, grid_x, stream};
     if constexpr (sizeof(T) == 4) {
         if (mf) return with_fast<T, true>(g.vec, g.m, fwd, lapx, LaunchOp<T>(op));
