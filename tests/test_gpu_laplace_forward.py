@@ -25,7 +25,8 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a
 # (dims, n, p, b, fused x pass expected in FP32)
 SHAPES = [(2, 256, 2, 8, False), (2, 1024, 2, 16, False), (2, 4096, 4, 16, True), (2, 4096, 2, 16, True),
           (2, 8192, 4, 32, True), (2, 8192, 2, 32, True), (3, 256, 2, 8, False), (3, 512, 2, 16, True),
-          (3, 512, 4, 8, True), (3, 512, 2, 8, True), (3, 256, 4, 4, False)]
+          (3, 512, 4, 8, True), (3, 512, 2, 8, True), (3, 256, 4, 4, False), (2, 4096, 1, 64, True),
+          (3, 256, 1, 16, True)]
 
 
 def note(msg):
