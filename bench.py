@@ -633,9 +633,9 @@ def main():
         extra["fp64_ms_per_step"] = round(ms64, 4)
         extra["fp64_stages"] = stage_roofline(st64, field * 16)
         extra["speedup_vs_fp64_gpu"] = round(ms64 / ms, 3)
-        # the north star's target: >= 3x the plain (fused) FP64 GPU path; both
-        # paths are HBM-bound and FP64 moves twice the bytes, so the ratio is
-        # bounded near 2 unless FP32 moves fewer sweeps than FP64
+        # the stated target: the FP32 step at >= 3x the plain FP64 GPU path;
+        # both paths are HBM-bound and FP64 moves twice the bytes, so the
+        # ratio is bounded near 2 unless FP32 moves fewer sweeps than FP64
         extra["target_vs_fp64_gpu"] = {"target": 3.0, "measured": round(ms64 / ms, 3), "met": ms64 / ms >= 3.0}
         log(f"errors vs FP64 GPU path: {json.dumps(errs)}")
         # the FP64 path with the reference's stage structure: per direction
