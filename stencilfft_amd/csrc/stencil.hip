@@ -341,7 +341,9 @@ __global__ void __launch_bounds__(256) deriv4_march_kernel(const E* __restrict__
 //   y: 2D K=2 with non-temporal loads, ring 8 / 6 ahead and T = 256 (FP64
 //      ring 16 / 12 ahead, T = 32, plain loads); 3D K=1, ring 16 / 12, T = 16
 //      (FP64 32) (2D 0.192 ms, 3D 0.408 ms; FP64 0.435, 0.827 ms);
-//   z: K=4, ring 8 / 6 ahead (FP64 5), T = 256 (0.419 ms; FP64 0.835 ms).
+//   z: K=4, ring 8 / 6 ahead (FP64 5), T = 256, FP32 with non-temporal loads
+//      (0.419 -> 0.407 ms; FP64 0.835 ms); 512 or 1024 threads with K = 2
+//      or 1, K = 8, and T = 64 or 128 were no faster.
 // Longer strips re-read fewer halo positions; shorter ones give more
 // workgroups: neither wins everywhere.
 // A K that does not divide the line count falls back to K=1.
@@ -383,7 +385,7 @@ static hipError_t launch_deriv4(const E* in, E* out, const Geo& g, int batch, in
     }
     const int lines = g.n * g.n;
     if (wide) return launch_march<E, S, 2, 4, 8, 5, false>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
-    return launch_march<E, S, 2, 4, 8, 6, false>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
+    return launch_march<E, S, 2, 4, 8, 6, true>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
 }
 
 // 5/7-point Laplacian as a rolling strip: a workgroup owns one column rank
