@@ -45,6 +45,17 @@ template <int LN>
 struct NumPasses {
     static constexpr int value = (LN + 3) / 4;
 };
+// SF: the smaller radix first (then radix 16), else radix 16 first; NS is the
+// Stockham stride of pass P (the product of the radices before it)
+template <int LN, int P, bool SF>
+struct RadixS {
+    static constexpr int first = (LN % 4) ? (1 << (LN % 4)) : 16;
+    static constexpr int value = SF ? (P == 0 ? first : 16) : Radix<LN, P>::value;
+};
+template <int LN, int P, bool SF>
+struct StrideS {
+    static constexpr int value = P == 0 ? 1 : SF ? RadixS<LN, 0, true>::value << (4 * (P - 1)) : 1 << (4 * P);
+};
 
 }  // namespace fast
 
@@ -118,7 +129,7 @@ __device__ __forceinline__ void fast_pass(const Cx<T>* src, Cx<T>* dst, const Cx
     constexpr int BPT = EPT / R;
     constexpr int ES = CONTIG ? 1 : VEC;
     static_assert(BPT * NT * R == E, "work split");
-    static_assert(NS == 1 || NS == 16, "per-pass tables cover the second pass of a stage");
+    static_assert(NS >= 1 && (NS & (NS - 1)) == 0 && NS <= 16, "per-pass tables cover the second pass of a stage");
     C v[BPT][R];
     int obase[BPT];
 #pragma unroll
@@ -364,13 +375,13 @@ struct NoSink {
 // OOP the passes alternate between the buffers `cur` and `alt`; the return
 // value is the buffer that holds the result.
 template <typename T, int LN_T, int PASS, int NPASS, int LVEC, int LN, int LE, int NT, bool CONTIG, bool FWD, bool PRE,
-          bool FULLTAB, bool MF, bool OOP, int HOFF = 0, bool FROMREG = false, typename Hook = NoHook>
+          bool FULLTAB, bool MF, bool OOP, int HOFF = 0, bool FROMREG = false, bool SF = false, typename Hook = NoHook>
 __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const Cx<T>* ts, const Cx<T>* ta,
                                                   const Cx<T>* tb, int tid, const Mfma16W* w, Hook&& hook = Hook{},
                                                   const Cx<T>* rin = nullptr) {
     if constexpr (PASS < NPASS) {
-        constexpr int R = fast::Radix<LN_T, PASS>::value;
-        constexpr int NS = 1 << (4 * PASS);
+        constexpr int R = fast::RadixS<LN_T, PASS, SF>::value;
+        constexpr int NS = fast::StrideS<LN_T, PASS, SF>::value;
         Cx<T>* dst = OOP ? alt : cur;
         static_assert(!(MF && FROMREG), "the MFMA passes read LDS");
         if constexpr (MF && R == 16)
@@ -381,8 +392,8 @@ __device__ __forceinline__ Cx<T>* fast_passes_lds(Cx<T>* cur, Cx<T>* alt, const 
                       kPerThreadPre<MF, FULLTAB, LN, LE>, FROMREG && PASS == 0>(cur, dst, ts, ta, tb, tid, NoSink{},
                                                                                rin);
         hook(std::integral_constant<int, HOFF + PASS + 1>{});
-        return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP, HOFF>(
-            dst, OOP ? cur : alt, ts, ta, tb, tid, w, hook);
+        return fast_passes_lds<T, LN_T, PASS + 1, NPASS, LVEC, LN, LE, NT, CONTIG, FWD, PRE, FULLTAB, MF, OOP, HOFF,
+                               false, SF>(dst, OOP ? cur : alt, ts, ta, tb, tid, w, hook);
     } else {
         return cur;
     }
@@ -432,11 +443,12 @@ __device__ __forceinline__ Cx<T> lap5(Cx<T> a, Cx<T> b, Cx<T> c, Cx<T> l, Cx<T> 
 // Whether the group goes from HBM straight into the first radix pass's
 // registers: vector-ALU inverse passes of one-line groups, whose first
 // (M-point) pass takes runs of VEC / P contiguous elements per rank across
-// the lanes.  Measured against staging through LDS on the same box: 8192^2
+// the lanes (and FP64 forward ones, below, SFV).  Measured against staging through LDS on the same box: 8192^2
 // FP32 0.211 -> 0.207 ms, FP64 0.395 -> 0.384 ms, 4096^2 FP32 unchanged.
-// Not the forward passes, whose first (VEC-point, stride VEC/16) pass would
-// take 32-byte pieces per lane group (8192^2 FP32 0.205 -> 0.291 ms), nor
-// the multi-line groups of 3D (512^3 FP32 inverse 0.416 -> 0.418 ms).
+// Not the forward passes with their radix-16 pass first, whose (VEC-point,
+// stride VEC/16) first pass would take 32-byte pieces per lane group (8192^2
+// FP32 0.205 -> 0.291 ms), nor the multi-line groups of 3D (512^3 FP32
+// inverse 0.416 -> 0.418 ms).
 template <typename T, bool MF, int LVEC, int LM, int LLPI, bool FWD>
 constexpr bool kDirectFirstPass = !MF && !FWD && LLPI == 0;
 
@@ -468,7 +480,14 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
     // DIRECT: the group is loaded from HBM straight into the order the first
     // radix pass takes it in, which then runs from registers: no staging
     // writes, reads or barriers
-    constexpr bool DIRECT = kDirectFirstPass<T, MF, LVEC, LM, LLPI, FWD>;
+    // SFV (FP64 forward passes of one-line groups): the VEC stage runs its
+    // smaller radix first, so that its first pass takes runs of 16
+    // contiguous elements across the lanes and can load the group straight
+    // into its registers (DIRECT).  8192^2 FP64 forward 0.416 -> 0.399 ms;
+    // in FP32 the forward pass was unchanged and the fused Laplacian pass
+    // (which stages its rows) slower (0.226 -> 0.230 ms).
+    constexpr bool SFV = FWD && !MF && sizeof(T) == 8 && LLPI == 0 && (LVEC % 4) != 0;
+    constexpr bool DIRECT = kDirectFirstPass<T, MF, LVEC, LM, LLPI, FWD> || SFV;
     C* lds = reinterpret_cast<C*>(smem_raw);
     C* lds2 = OOP ? lds + fast_padded<T, LN>(E) : lds;
     C* tabs = lds + fast_padded<T, LN>(E) * (OOP ? 2 : 1);
@@ -481,7 +500,10 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
     // the second pass (NS = 16, radix R) of the VEC and M stages.
     constexpr bool PTW = kPerThreadPre<MF, FULLTAB, LN, LE>;
     constexpr int TV = 0, TM = VEC, TP = VEC + M;
-    if constexpr (NPV > 1) {
+    if constexpr (NPV > 1 && SFV) {
+        constexpr int R0 = fast::RadixS<LVEC, 0, true>::value, L0 = fast::Log2<R0>::value;
+        for (int t = tid; t < VEC; t += NT) tabs[TV + t] = tables[(t & (R0 - 1)) * (t >> L0) * (S / VEC)];
+    } else if constexpr (NPV > 1) {
         constexpr int RV = fast::Radix<LVEC, 1>::value;
         for (int t = tid; t < 16 * RV; t += NT) tabs[TV + t] = tables[(t & 15) * (t >> 4) * (S / (16 * RV))];
     }
@@ -555,7 +577,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
     // map below)
     auto slot_li = [&](int k) -> int {
         if constexpr (FWD) {
-            constexpr int R0 = fast::Radix<LVEC, 0>::value, nbf = VEC / R0, LNBF = fast::Log2<nbf>::value;
+            constexpr int R0 = fast::RadixS<LVEC, 0, SFV>::value, nbf = VEC / R0, LNBF = fast::Log2<nbf>::value;
             const int u = tid + (k / R0) * NT, r = k % R0;
             return ((u >> LNBF) << LVEC) + (u & (nbf - 1)) + r * nbf;
         } else {
@@ -641,8 +663,8 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
 
         if constexpr (FWD) {
             // VEC-point DFTs (contiguous), all passes in LDS
-            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0, FR>(
-                lds, lds2, tsv, ta, tb, tid, w, hook, pre);
+            C* cur = fast_passes_lds<T, LVEC, 0, NPV, LVEC, LN, LE, NT, true, true, false, FULLTAB, MF, OOP, 0, FR,
+                                     SFV>(lds, lds2, tsv, ta, tb, tid, w, hook, pre);
             // M-point DFTs (stride VEC) with the inter-stage twiddle on the
             // first pass; the last pass stores to HBM
             if constexpr (STAGED) {
