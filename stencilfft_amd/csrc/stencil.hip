@@ -543,7 +543,7 @@ __global__ void __launch_bounds__(256) laplace_strip_kernel(const E* __restrict_
 // fetched into the same L2.  Against the row strip kernel, which re-reads the rows of
 // planes gd +- 1 from L2 (FETCH 1.32x the field at 512^3), this took the
 // 512^3 FP32 Laplacian from 0.60 to 0.41 ms (0.40 with 512 threads), and
-// the FP64 one from 0.93 to 0.90 ms with 512 threads.
+// the FP64 one from 0.93 to 0.79 ms (1024^3 at L = 512: 8.05 to 6.75 ms).
 // NTH threads per workgroup; KE = R L / NTH tile elements per thread; KY, HX:
 // y-halo and x-halo elements per thread (guarded).
 template <typename E, typename S, int R, int KE, int KY, int HX, int NTH = 256>
@@ -588,28 +588,35 @@ __global__ void __launch_bounds__(NTH) laplace_zmarch_kernel(const E* __restrict
             c[k] = src[row_off(z, r0 + (e >> g.ll), cid) + (e & (L - 1))];
         }
     };
+    // The guarded loads go to a temporary and the slot is written on every
+    // path: with the slot written under the guard, the double2 instantiations
+    // kept the halo slots in scratch memory and waited on each halo load.
     auto load_halo = [&](int z, E* yh, E* xh) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < KY; k++) {
             const int e = tid + NTH * k;
+            E v = O::zero();
             if (e < nyh) {
                 const int side = e >> g.ll;  // 0: row r0 - 1, 1: row r0 + R
-                yh[k] = src[row_off(z, side ? r0 + R : r0 - 1, cid) + (e & (L - 1))];
+                v = src[row_off(z, side ? r0 + R : r0 - 1, cid) + (e & (L - 1))];
             }
+            yh[k] = v;
         }
 #pragma unroll
         for (int k = 0; k < HX; k++) {
             const int e = tid + NTH * k;
+            E v = O::zero();
             if (e < nxh) {
                 const int row = e >> (g.lvec + 1), side = (e >> g.lvec) & 1, jj = e & (VEC - 1);
                 if (side == 0) {
                     const int jl = (cid == 0) ? ((jj - 1 + VEC) & (VEC - 1)) : jj;
-                    xh[k] = src[row_off(z, r0 + row, cl) + (B - 1) * VEC + jl];
+                    v = src[row_off(z, r0 + row, cl) + (B - 1) * VEC + jl];
                 } else {
                     const int jr = (cid == P - 1) ? ((jj + 1) & (VEC - 1)) : jj;
-                    xh[k] = src[row_off(z, r0 + row, cr) + jr];
+                    v = src[row_off(z, r0 + row, cr) + jr];
                 }
             }
+            xh[k] = v;
         }
     };
     E pl[4][KE], yh[2][KY > 0 ? KY : 1], xh[2][HX > 0 ? HX : 1];
@@ -764,7 +771,7 @@ static hipError_t laplace_zmarch(const E* in, E* out, const Geo& g, int batch, d
         case 32: return laplace_zmarch_inst<E, S, 8, 1, 1>(in, out, g, batch, coef, stream);
         case 64: return laplace_zmarch_inst<E, S, 8, 2, 1>(in, out, g, batch, coef, stream);
         case 128: return laplace_zmarch_inst<E, S, 8, 4, 1>(in, out, g, batch, coef, stream);
-        // 512 threads: 0.405 -> 0.399 ms (FP32), 0.95 -> 0.90 ms (FP64) at 512^3
+        // 512 threads: 0.405 -> 0.399 ms (FP32) at 512^3
         case 256: return laplace_zmarch_inst<E, S, 8, 4, 1, 512>(in, out, g, batch, coef, stream);
         case 512: return laplace_zmarch_inst<E, S, 4, 8, 4>(in, out, g, batch, coef, stream);
         case 1024: return laplace_zmarch_inst<E, S, 2, 8, 8>(in, out, g, batch, coef, stream);
@@ -779,11 +786,8 @@ static hipError_t stencil_dispatch(const void* in, void* out, const Geo& g, int 
     const dim3 grid((unsigned)((g.field + threads - 1) / threads), (unsigned)batch);
     const E* src = static_cast<const E*>(in);
     E* dst = static_cast<E*>(out);
-    // 3D: the depth march, except FP64 with L != 256 where the strip kernel
-    // applies (the 256-thread march measured slower there; L = 256 runs the
-    // 512-thread march, 0.90 against 0.93 ms for the strip kernel at 512^3)
-    constexpr bool fp64 = sizeof(S) == 8;
-    if (op == OP_LAPLACE && zmarch_ok(g, sizeof(E)) && !(fp64 && g.l != 256 && strip_ok(g, batch))) {
+    // 3D: the depth march where its tile fits, else the strip kernel
+    if (op == OP_LAPLACE && zmarch_ok(g, sizeof(E))) {
         return laplace_zmarch<E, S>(src, dst, g, batch, coef, stream);
     } else if (op == OP_LAPLACE && strip_ok(g, batch)) {
         return laplace_strip<E, S>(src, dst, g, batch, coef, stream);

@@ -61,7 +61,7 @@ def kernels(tmp_path_factory):
             if not nm:
                 continue
             raw[nm.group(1)] = {k: int(v) for k, v in re.findall(
-                r"\.(vgpr_spill_count|sgpr_spill_count|vgpr_count):\s+(\d+)", entry)}
+                r"\.(vgpr_spill_count|sgpr_spill_count|vgpr_count|private_segment_fixed_size):\s+(\d+)", entry)}
     assert raw, "no kernel metadata found"
     mangled = list(raw)
     names = subprocess.run([tools["demangle"]], input="\n".join(mangled), check=True, capture_output=True,
@@ -82,6 +82,18 @@ def test_fused_path_kernels_do_not_spill(kernels):
     unexpected = {k: m for k, m in spilled.items() if KNOWN.get(k, -1) < m.get("vgpr_spill_count", 0)
                   or m.get("sgpr_spill_count", 0)}
     assert not unexpected, unexpected
+
+
+def test_no_kernel_keeps_arrays_in_scratch(kernels):
+    # A private segment without spills means a per-thread array the compiler
+    # left in scratch memory: its accesses go through the cache hierarchy to
+    # HBM and each load waits before the next step.  The double2 depth-march
+    # Laplacian once held its halo slots there (80-336 B per thread), which
+    # wrote 1.375x and read 1.64x the field at 512^3 FP64.  Only the two known
+    # spilling kernels may use scratch, for their spill slots.
+    scratch = {k: m["private_segment_fixed_size"] for k, m in kernels.items()
+               if m.get("private_segment_fixed_size", 0) and k not in KNOWN}
+    assert not scratch, scratch
 
 
 def test_fp64_line_pass_at_n8192_fits_without_spills(kernels):
