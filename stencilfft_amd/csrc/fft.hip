@@ -298,15 +298,18 @@ struct Vec16<double> {
     __device__ static double2 join(const double2* c) { return c[0]; }
 };
 
-template <typename T, bool FWD, int LTX, int LTY>
+template <typename T, bool FWD, int LTX, int LTY, int LKR = 0>
 __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restrict__ in, Cx<T>* __restrict__ out,
                                                            Geo g, RepackArgs a, int lecount, int order) {
     using C = Cx<T>;
     using V16 = Vec16<T>;
     using Vt = typename V16::type;
-    constexpr int TX = 1 << LTX, TY = 1 << LTY, V = V16::n;
-    constexpr int XA = TY / V, XR = 256 / XA;  // x/y side: accesses per row, rows per sweep
-    constexpr int PA = TX / V, PR = 256 / PA;  // packed side
+    // KR = 2^LKR consecutive rho per tile (used with TX = VEC only): the
+    // tile's TXE = KR TX rows are x = rho0 + j + B (t0 + t), j < KR, t < TX,
+    // whose packed positions (rho0 + j) VEC + t form one run of KR VEC values
+    constexpr int TX = 1 << LTX, TY = 1 << LTY, V = V16::n, KR = 1 << LKR, TXE = TX * KR;
+    constexpr int XA = TY / V, XR = 256 / XA;   // x/y side: accesses per row, rows per sweep
+    constexpr int PA = TXE / V, PR = 256 / PA;  // packed side
     static_assert(XR >= 1 && TX % XR == 0 && PR >= 1 && TY % PR == 0, "tile split");
     // non-temporal loads; non-temporal stores for FP64 only (FP32 plain
     // stores measured faster over whole steps, line pass included)
@@ -315,16 +318,16 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
         if constexpr (sizeof(T) == 8) st_nt(p, v);
         else *p = v;
     };
-    __shared__ C tile[TX][TY + 1];
+    __shared__ C tile[TXE][TY + 1];
     const int tid = threadIdx.x;
     // block order (`order`, see launch_repack): bit 0 XCD-aware, bit 1 rho
     // fastest, else y tile fastest
     unsigned rest = (order & 1) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int lyb = g.ll - LTY, ltb = g.lvec - LTX;
+    const int lyb = g.ll - LTY, ltb = g.lvec - LTX, lrb = g.lb - LKR;
     int yb, tb, rho;
     if (order & 2) {
-        rho = (int)(rest & (unsigned)(g.b - 1));
-        rest >>= g.lb;
+        rho = (int)(rest & ((1u << lrb) - 1)) << LKR;
+        rest >>= lrb;
         yb = (int)(rest & ((1u << lyb) - 1));
         rest >>= lyb;
         tb = (int)(rest & ((1u << ltb) - 1));
@@ -334,28 +337,32 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
         rest >>= lyb;
         tb = (int)(rest & ((1u << ltb) - 1));
         rest >>= ltb;
-        rho = (int)(rest & (unsigned)(g.b - 1));
-        rest >>= g.lb;
+        rho = (int)(rest & ((1u << lrb) - 1)) << LKR;
+        rest >>= lrb;
     }
     const int e = (int)(rest & ((1u << lecount) - 1));
     const int id = (int)(rest >> lecount);
     const C* src = in + (int64_t)blockIdx.y * g.field + (int64_t)id * g.local;
     C* dst = out + (int64_t)blockIdx.y * g.field + (int64_t)id * g.local;
     const int t0 = tb << LTX, y0 = yb << LTY;
-    // x/y side: row t holds x = rho + B*(t0 + t), columns y0 .. y0 + TY - 1
+    // x/y side: tile row p = j TX + t holds x = rho + j + B*(t0 + t),
+    // columns y0 .. y0 + TY - 1; sweep i covers rows xr + i XR, so its j and
+    // t offsets are compile-time
     const int xc = (tid % XA) * V, xr = tid / XA;
     const C* xy = (FWD ? src : dst) + (int64_t)e * a.ei + y0 + xc;
-    // packed side: row y0 + y holds pack(x) = rho*VEC + t0 .. + TX - 1
+    // packed side: row y0 + y holds pack(x) = rho*VEC + t0 .. + TXE - 1
     const int pc = (tid % PA) * V, pr = tid / PA;
     const C* pk = (FWD ? dst : src) + (int64_t)e * a.eo + (int64_t)y0 * a.syo + (rho << g.lvec) + t0 + pc;
     const int64_t xstep = (int64_t)g.b * a.sxi;
     const C* xyr = xy + (int64_t)(rho + g.b * (t0 + xr)) * a.sxi;
+    constexpr int SW = TX / XR;  // sweeps per rho
+    auto xrow = [&](int i) { return xyr + (int64_t)(i / SW) * a.sxi + (int64_t)(i % SW) * XR * xstep; };
     if constexpr (FWD) {
-        Vt r[TX / XR];
+        Vt r[TXE / XR];
 #pragma unroll
-        for (int i = 0; i < TX / XR; i++) r[i] = ldv(reinterpret_cast<const Vt*>(xyr + (int64_t)i * XR * xstep));
+        for (int i = 0; i < TXE / XR; i++) r[i] = ldv(reinterpret_cast<const Vt*>(xrow(i)));
 #pragma unroll
-        for (int i = 0; i < TX / XR; i++) {
+        for (int i = 0; i < TXE / XR; i++) {
             C c[V];
             V16::split(r[i], c);
 #pragma unroll
@@ -383,12 +390,12 @@ __global__ void __launch_bounds__(256) repack_tiled_kernel(const Cx<T>* __restri
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < TX / XR; i++) {
+        for (int i = 0; i < TXE / XR; i++) {
             const int t = xr + i * XR;
             C c[V];
 #pragma unroll
             for (int v = 0; v < V; v++) c[v] = tile[t][xc + v];
-            stv(reinterpret_cast<Vt*>(const_cast<C*>(xyr) + (int64_t)i * XR * xstep), V16::join(c));
+            stv(reinterpret_cast<Vt*>(const_cast<C*>(xrow(i))), V16::join(c));
         }
     }
 }
@@ -494,20 +501,25 @@ hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const Repack
         // fastest, XCD-aware (0.402 / 0.465 -> 0.377 / 0.387 ms); 3D FP64
         // keeps the plain order (XCD-aware: 0.739 -> 0.756 ms).
         const int order = g.dims == 2 ? (sizeof(T) == 8 ? 3 : 2) : (sizeof(T) == 8 ? 0 : 1);
+        // VEC = 16: two rho per tile, so that the packed side writes runs of
+        // 2 VEC values (256 bytes at FP32) like the 2D VEC >= 32 shapes
+        const int lkr = (g.vec == 16 && g.b >= 2) ? 1 : 0;
         const unsigned blocks =
-            (unsigned)((int64_t)g.nranks * a.ecount * g.b * (g.vec >> ltx) * (g.l >> lty));
+            (unsigned)((int64_t)g.nranks * a.ecount * (g.b >> lkr) * (g.vec >> ltx) * (g.l >> lty));
         const dim3 grid(blocks, (unsigned)batch);
-#define ZM_REPACK(F, X, Y)                                                                                     \
-    if (fwd == F && ltx == X && lty == Y) {                                                                    \
-        hipLaunchKernelGGL((repack_tiled_kernel<T, F, X, Y>), grid, dim3(256), 0, stream, in, out, g, a, lecount, \
-                           order);                                                                             \
-        return hipGetLastError();                                                                              \
+#define ZM_REPACK(F, X, Y, K)                                                                                   \
+    if (fwd == F && ltx == X && lty == Y && lkr == K) {                                                         \
+        hipLaunchKernelGGL((repack_tiled_kernel<T, F, X, Y, K>), grid, dim3(256), 0, stream, in, out, g, a,      \
+                           lecount, order);                                                                     \
+        return hipGetLastError();                                                                               \
     }
         if constexpr (sizeof(T) == 4) {
-            ZM_REPACK(true, 5, 7) ZM_REPACK(true, 4, 7) ZM_REPACK(false, 5, 7) ZM_REPACK(false, 4, 7)
+            ZM_REPACK(true, 5, 7, 0) ZM_REPACK(true, 4, 7, 0) ZM_REPACK(false, 5, 7, 0) ZM_REPACK(false, 4, 7, 0)
+            ZM_REPACK(true, 4, 7, 1) ZM_REPACK(false, 4, 7, 1)
         }
-        ZM_REPACK(true, 5, 6) ZM_REPACK(true, 5, 5) ZM_REPACK(true, 4, 6) ZM_REPACK(true, 4, 5)
-        ZM_REPACK(false, 5, 6) ZM_REPACK(false, 5, 5) ZM_REPACK(false, 4, 6) ZM_REPACK(false, 4, 5)
+        ZM_REPACK(true, 5, 6, 0) ZM_REPACK(true, 5, 5, 0) ZM_REPACK(true, 4, 6, 0) ZM_REPACK(true, 4, 5, 0)
+        ZM_REPACK(false, 5, 6, 0) ZM_REPACK(false, 5, 5, 0) ZM_REPACK(false, 4, 6, 0) ZM_REPACK(false, 4, 5, 0)
+        ZM_REPACK(true, 4, 6, 1) ZM_REPACK(true, 4, 5, 1) ZM_REPACK(false, 4, 6, 1) ZM_REPACK(false, 4, 5, 1)
 #undef ZM_REPACK
     }
     const unsigned blocks = (unsigned)(g.nranks * a.ecount * g.b * (g.vec / a.tx) * (g.l / a.ty));
