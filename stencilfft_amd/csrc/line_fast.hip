@@ -897,13 +897,17 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
         const int tidh = threadIdx.x;
         auto elem = [&](int k2) -> int { return tid + NT * k2; };
         auto elemh = [&](int k2) -> int { return tidh + NT * k2; };
+        // The tile's first and last rows are the neighbouring tiles' halo
+        // rows: plain loads keep them in L2 for those tiles (which run on the
+        // same XCD), where non-temporal ones had them re-read from HBM.
         auto fetch_plane = [&](C* dst, int gz, auto k0c, auto k1c) __attribute__((always_inline)) {
             constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
             const C* gb = src0 + group_base(group_of(gz));
 #pragma unroll
             for (int k = K0; k < K1; k++) {
                 const int l = k / KP, e = elem(k % KP);
-                dst[k] = ldg(gb + off(e >> ll, l, e & (L - 1)));
+                const C* p = gb + off(e >> ll, l, e & (L - 1));
+                dst[k] = (l == 0 || l == LPI - 1) ? *p : ldg(p);
             }
         };
         C up[EPT], rowa[EPT], rowb[EPT], hal[2][2][KP];
