@@ -21,6 +21,10 @@ torch = pytest.importorskip("torch")
 
 SHAPES_2D = [(16, 2, 2), (32, 2, 4), (64, 4, 2), (64, 2, 1), (32, 2, 8), (64, 4, 4), (128, 2, 16), (256, 4, 8)]
 SHAPES_3D = [(16, 2, 2), (16, 2, 1), (32, 2, 4), (32, 4, 2)]
+# VEC >= 16 and L >= 32: the repack's compile-time tiles (two rho per tile at
+# VEC = 16, one at VEC = 32; y tiles of 32, 64 and 128)
+TILED_SHAPES = [(2, 64, 2, 2), (2, 128, 2, 4), (2, 256, 2, 4), (2, 512, 4, 8), (3, 64, 2, 2), (3, 64, 1, 4),
+                (3, 128, 2, 4), (3, 128, 2, 2)]
 PATHS = [False, True]  # fused line kernel, two-stage
 
 
@@ -67,7 +71,7 @@ def test_permute_matches_images(precision, two_stage):
 
 
 @pytest.mark.parametrize("two_stage", PATHS)
-@pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D])
+@pytest.mark.parametrize("dims,n,p,b", [(2,) + s for s in SHAPES_2D] + [(3,) + s for s in SHAPES_3D] + TILED_SHAPES)
 def test_permute_bit_exact_vs_oracle(oracle_mod, dims, n, p, b, two_stage):
     note(f"permute {dims}D N={n} P={p} B={b} two_stage={two_stage}")
     s = oracle_mod.Shape(dims, n, p, b)
