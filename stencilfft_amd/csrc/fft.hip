@@ -483,36 +483,49 @@ hipError_t launch_stage_b(const Cx<T>* in, Cx<T>* out, const Geo& g, const Stage
 template <typename T>
 hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const RepackArgs& a, int batch, bool fwd,
                          hipStream_t stream) {
-    // tiled path: VEC >= 16 and L >= 32 (x tile 32 or 16; y tile 128 for
-    // FP32 when L >= 128, else 64 or 32: the largest measured fastest, and a
-    // 32 x 128 tile of complex128 would not fit static LDS)
+    // tiled path: VEC >= 16 and L >= 32.  A tile is TX = 2^ltx values of x
+    // (times KR = 2^lkr consecutive rho) by TY = 2^lty values of y; `order`
+    // is the block order (see repack_tiled_kernel).  From same-box sweeps of
+    // x tile (16, 32), y tile (32-128), rho per tile and order at 8192^2
+    // (P = 4, B = 32) and 512^3 (P = 2, B = 16):
+    // * 2D FP32: 16 x 32 tiles, rho fastest, so that the blocks in flight
+    //   together write whole packed rows.  32 x 128 tiles in that order had
+    //   taken forward / inverse 0.203 / 0.197 -> 0.170 / 0.174 ms against y
+    //   tile fastest; 16 x 32: 0.171 -> 0.152 ms (7.1 TB/s), step 1.187 ->
+    //   1.139 ms.
+    // * 2D FP64: 32 x 64, rho fastest (0.364 -> 0.354 ms against the same
+    //   XCD-aware).
+    // * 3D FP32: 16 (or 32) x 32 tiles of one rho, y tile fastest,
+    //   XCD-aware.  A tile of the xy repack then spans 32 rows of one plane,
+    //   one of the yz repack 32 + 16 planes instead of 128 + 32: 0.387 /
+    //   0.382 -> 0.356 / 0.352 ms (y tile fastest in plain order had taken
+    //   0.402 / 0.465 ms).
+    // * 3D FP64: plain order, two rho per tile at VEC = 16 (XCD-aware and
+    //   other tiles no better).
+    // * VEC = 16 otherwise: two rho per tile, so that the packed side writes
+    //   runs of 2 VEC values like the VEC >= 32 shapes (4096^2, P = 4,
+    //   B = 64, 32 x 128 tiles: 0.0586 / 0.0574 -> 0.0567 / 0.0542 ms).
     if (g.vec >= 16 && g.l >= 32) {
-        const int ltx = g.vec >= 32 ? 5 : 4;
-        int lty = (sizeof(T) == 4 && g.l >= 128) ? 7 : (g.l >= 64 ? 6 : 5);
-        int lecount = 0;
-        while ((1 << lecount) < a.ecount) lecount++;
-        // Block order, from a same-box sweep of orders at 8192^2 (P = 4 and
-        // 2) and 512^3.  2D: rho fastest, so that the blocks in flight
-        // together write whole packed rows (B runs of VEC values) instead of
-        // 256-byte pieces of 128 rows each: FP32 P=4 forward / inverse 0.203
-        // / 0.197 -> 0.170 / 0.174 ms (step 1.253 -> 1.218 ms, the line pass
-        // after the inverse repack losing 0.006 ms of it); FP64 also
-        // XCD-aware (0.378 / 0.412 -> 0.340 / 0.357 ms).  3D FP32: y tile
-        // fastest, XCD-aware (0.402 / 0.465 -> 0.377 / 0.387 ms); 3D FP64
-        // keeps the plain order (XCD-aware: 0.739 -> 0.756 ms).
-        const int order = g.dims == 2 ? (sizeof(T) == 8 ? 3 : 2) : (sizeof(T) == 8 ? 0 : 1);
-        // VEC = 16: two rho per tile, so that the packed side writes runs of
-        // 2 VEC values (256 bytes at FP32) like the 2D VEC >= 32 shapes
+        constexpr bool fp32 = sizeof(T) == 4;
+        int ltx = g.vec >= 32 ? 5 : 4;
+        int lty = g.l >= 64 ? 6 : 5;
         int lkr = (g.vec == 16 && g.b >= 2) ? 1 : 0;
-        // 3D FP32: tiles of 32 y by one rho.  A tile of the xy repack then
-        // spans 32 rows of one plane, one of the yz repack 32 + 16 planes
-        // instead of 128 + 32; from a same-box sweep of y tile, rho per tile
-        // and order at 512^3: 0.387 / 0.383 -> 0.371 / 0.366 ms (forward /
-        // inverse, each alone); FP64 showed no gain beyond noise
-        if (g.dims == 3 && sizeof(T) == 4) {
+        int order;
+        if (g.dims == 2) {
+            order = 2;
+            if (fp32) {
+                ltx = 4;
+                lty = 5;
+            }
+        } else if (fp32) {
+            order = 1;
             lty = 5;
             lkr = 0;
+        } else {
+            order = 0;
         }
+        int lecount = 0;
+        while ((1 << lecount) < a.ecount) lecount++;
         const unsigned blocks =
             (unsigned)((int64_t)g.nranks * a.ecount * (g.b >> lkr) * (g.vec >> ltx) * (g.l >> lty));
         const dim3 grid(blocks, (unsigned)batch);
@@ -522,10 +535,6 @@ hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const Repack
                            lecount, order);                                                                     \
         return hipGetLastError();                                                                               \
     }
-        if constexpr (sizeof(T) == 4) {
-            ZM_REPACK(true, 5, 7, 0) ZM_REPACK(true, 4, 7, 0) ZM_REPACK(false, 5, 7, 0) ZM_REPACK(false, 4, 7, 0)
-            ZM_REPACK(true, 4, 7, 1) ZM_REPACK(false, 4, 7, 1)
-        }
         ZM_REPACK(true, 5, 6, 0) ZM_REPACK(true, 5, 5, 0) ZM_REPACK(true, 4, 6, 0) ZM_REPACK(true, 4, 5, 0)
         ZM_REPACK(false, 5, 6, 0) ZM_REPACK(false, 5, 5, 0) ZM_REPACK(false, 4, 6, 0) ZM_REPACK(false, 4, 5, 0)
         ZM_REPACK(true, 4, 6, 1) ZM_REPACK(true, 4, 5, 1) ZM_REPACK(false, 4, 6, 1) ZM_REPACK(false, 4, 5, 1)
