@@ -524,6 +524,9 @@ hipError_t launch_repack(const Cx<T>* in, Cx<T>* out, const Geo& g, const Repack
         } else {
             order = 0;
         }
+        // several rho per tile form one packed run only when the x tile is
+        // the whole vector (t0 = 0)
+        if ((1 << ltx) != g.vec) lkr = 0;
         int lecount = 0;
         while ((1 << lecount) < a.ecount) lecount++;
         const unsigned blocks =
