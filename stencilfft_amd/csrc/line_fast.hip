@@ -418,9 +418,12 @@ __device__ __forceinline__ void last_pass(const Cx<T>* cur, const Cx<T>* ts, con
 // shift at the global x edge, laplace.cu:148, 160).  FP64 sums in the
 // reference's order (laplace.cu:117-167), FP32 in the first-difference order
 // of the stencil kernels, both then times `scale`: the same operations as
-// zm_laplace, so the values are bit for bit its own.
+// zm_laplace, so the values are bit for bit its own.  The multiply by
+// `scale` is unconditional: an unscaled call passes 1, and x * 1 is exact
+// (the kernels keep denormals), where a select per value cost 32 VALU
+// instructions per thread and group.
 template <typename T>
-__device__ __forceinline__ Cx<T> lap5(Cx<T> a, Cx<T> b, Cx<T> c, Cx<T> l, Cx<T> r, T scale, bool apply_scale) {
+__device__ __forceinline__ Cx<T> lap5(Cx<T> a, Cx<T> b, Cx<T> c, Cx<T> l, Cx<T> r, T scale) {
 #pragma clang fp contract(off)
     using C = Cx<T>;
     C s;
@@ -436,7 +439,7 @@ __device__ __forceinline__ Cx<T> lap5(Cx<T> a, Cx<T> b, Cx<T> c, Cx<T> l, Cx<T> 
         const T xx = (l.x - c.x) + (r.x - c.x), xy = (l.y - c.y) + (r.y - c.y);
         s = C{yx + xx, yy + xy};
     }
-    if (apply_scale) s = C{s.x * scale, s.y * scale};
+    s = C{s.x * scale, s.y * scale};
     return s;
 }
 
@@ -461,7 +464,7 @@ constexpr bool kDirectFirstPass = !MF && !FWD && LLPI == 0;
 template <typename T, int LVEC, int LM, int LLPI, bool FWD, bool FULLTAB, bool MF, bool LAPX = false>
 __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI > 0) ? 2 : 1)
     line_fast_kernel(const Cx<T>* in, Cx<T>* out, Geo g, LineArgs a, const Cx<T>* __restrict__ tables, T scale,
-                     int apply_scale, int groups) {
+                     int groups) {
     using C = Cx<T>;
     constexpr int VEC = 1 << LVEC, M = 1 << LM;
     constexpr int LN = LVEC + LM, N = 1 << LN;
@@ -716,7 +719,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
                     const int e = tid + k * NT;
                     const int l = e >> LN, rem = e & (N - 1);
                     C v = cur[fidx<T, LN>(e)];
-                    if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
+                    v = C{v.x * sc, v.y * sc};
                     stg(gd + off(rem >> ll, l, rem & (L - 1)), v);
                 }
             } else {
@@ -732,7 +735,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
                         const int bt = u >> fast::Log2<nbf>::value;
                         const int r = bt & (M - 1), line = bt >> LM;
                         const int gm = r >> g.lb, j = r & (g.b - 1);
-                        if (MF || apply_scale) v = C{v.x * sc, v.y * sc};
+                        v = C{v.x * sc, v.y * sc};
                         stg(gd + off(gm, line, (j << LVEC) + jj), v);
                     });
             }
@@ -817,7 +820,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
 #pragma unroll
             for (int k = 0; k < EPT; k++) {
                 const C l = k > 0 ? x[k - 1] : lft, r = k < EPT - 1 ? x[k + 1] : rgt;
-                lds[fidx<T, LN>(elem(k))] = lap5<T>(up[k], y[k], x[k], l, r, scale, apply_scale != 0);
+                lds[fidx<T, LN>(elem(k))] = lap5<T>(up[k], y[k], x[k], l, r, scale);
             }
 #pragma unroll
             for (int k = 0; k < EPT; k++) up[k] = x[k];
@@ -972,7 +975,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
                 const T sx = (xa.x - c.x) + (xb.x - c.x), tx = (xa.y - c.y) + (xb.y - c.y);
                 const T sz = (za.x - c.x) + (zb.x - c.x), tzz = (za.y - c.y) + (zb.y - c.y);
                 C s = C{(sy + sx) + sz, (ty + tx) + tzz};
-                if (apply_scale) s = C{s.x * scale, s.y * scale};
+                s = C{s.x * scale, s.y * scale};
                 lds[fidx<T, LN>((l << LN) + elem(k2))] = s;
             }
 #pragma unroll
@@ -1126,7 +1129,7 @@ static hipError_t launch_fast_inst(const Cx<T>* in, Cx<T>* out, const Geo& g, co
     const int groups = (int)(g.field / N) >> LLPI;
     const dim3 grid((unsigned)(grid_x < groups ? grid_x : groups), (unsigned)batch);
     hipLaunchKernelGGL((line_fast_kernel<T, LVEC, LM, LLPI, FWD, FULLTAB, MF, LAPX>), grid, dim3(E / EPT), lds, stream,
-                       in, out, g, a, tables, (T)scale, scale != 1.0 ? 1 : 0, groups);
+                       in, out, g, a, tables, (T)scale, groups);
     return hipGetLastError();
 }
 
