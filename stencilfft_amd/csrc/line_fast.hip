@@ -1006,7 +1006,16 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
         return;
     }
 
-    int grp = blockIdx.x;
+    // Groups run strided over the workgroups (grp = wg + k gridDim.x), so the
+    // lines in flight together are consecutive; contiguous chunks per
+    // workgroup were slower (8192^2 FP32 forward / inverse 0.200 / 0.205 ->
+    // 0.218 / 0.243 ms).  FP32 one-line groups of N = 8192 number the
+    // workgroups XCD-aware, so that the 32 workgroups of an XCD take 32
+    // consecutive lines: 0.199 / 0.205 -> 0.194 / 0.200 ms; FP64 at N = 8192
+    // (forward 0.397 -> 0.414 ms), FP32 at N = 4096 and the 3D groups were
+    // slower that way.
+    constexpr bool kXcdLines = sizeof(T) == 4 && !MF && LLPI == 0 && LN == 13;
+    int grp = kXcdLines ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     int it = 0;
     if (grp < groups) {
         fetch(grp);
