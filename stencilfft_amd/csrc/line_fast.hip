@@ -1013,7 +1013,7 @@ __global__ void __launch_bounds__((1 << (LVEC + LM + LLPI)) / EPT, (LAPX && LLPI
     // workgroups XCD-aware, so that the 32 workgroups of an XCD take 32
     // consecutive lines: 0.199 / 0.205 -> 0.194 / 0.200 ms; FP64 at N = 8192
     // (forward 0.397 -> 0.414 ms), FP32 at N = 4096 and the 3D groups were
-    // slower that way.
+    // slower that way, and the MFMA pass at N = 8192 the same.
     constexpr bool kXcdLines = sizeof(T) == 4 && !MF && LLPI == 0 && LN == 13;
     int grp = kXcdLines ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     int it = 0;
