@@ -274,6 +274,19 @@ def test_laplace_3d_depth_march(oracle_mod, n, p, b):
     assert rel_l2(g32, s.laplace(x.astype(np.complex64).astype(np.complex128))) < 1e-6
 
 
+def test_laplace_3d_depth_march_rows_of_512_fp64(oracle_mod):
+    """512^3, P=1, B=32: rows of L = 512 packed columns, the depth march's
+    tiles of 4 rows (FP64 took the strip kernel at this L before the march's
+    halo slots left scratch memory); FP64 bit for bit against the oracle."""
+    n, p, b = 512, 1, 32
+    note(f"depth-march laplace 3D N={n} P={p} B={b} fp64")
+    s = oracle_mod.Shape(3, n, p, b)
+    x = uniform_field(n ** 3, 901)  # any field in the packed layout
+    pl = plan(3, n, p, b, "fp64")
+    got = host(pl.laplace(dev(x, torch.complex128), scale=3.0))
+    assert np.array_equal(got.view(np.int64), (s.laplace(x) * 3.0).view(np.int64))
+
+
 @pytest.mark.parametrize("n,p,b", [(16, 2, 2), (32, 2, 4), (64, 4, 2), (32, 2, 8), (64, 2, 2), (256, 4, 8)])
 def test_dx_dy_fp64_bit_exact(oracle_mod, n, p, b):
     note(f"dx/dy N={n} P={p} B={b}")
