@@ -224,18 +224,22 @@ __device__ __forceinline__ E deriv4_combine(E fm1, E fm2, E fp1, E fp2, S denom,
 // RING-slot register ring that runs AHEAD positions ahead of the output: each
 // value is loaded once per strip (plus 4 halo loads at the strip ends),
 // (AHEAD - 4) K loads are in flight while one position is computed, and no
-// value is exchanged between threads.  Strips of one line are consecutive
-// blocks in XCD-aware order, so a strip's halo loads hit the L2 that its
-// neighbour strip's loads filled.  NTL: non-temporal loads.
+// value is exchanged between threads.  Block order (`order`): 0, strips of
+// one line set are consecutive blocks in XCD-aware order, so a strip's halo
+// loads hit the L2 that its neighbour strip's loads filled; 1, the same in
+// plain order; 2, XCD-aware with consecutive line sets (the blocks in flight
+// together cover adjacent columns at one strip).  NTL: non-temporal loads.
 template <typename E, typename S, int AXIS, int K, int RING, int AHEAD, bool NTL>
 __global__ void __launch_bounds__(256) deriv4_march_kernel(const E* __restrict__ in, E* __restrict__ out, Geo g,
-                                                           int T, int lstrips, int lines, S denom, int negate) {
+                                                           int T, int lstrips, int lines, S denom, int negate,
+                                                           int order) {
 #pragma clang fp contract(off)
     using O = EOps<E>;
     static_assert(AHEAD < RING && AHEAD > 4, "the slot loaded must hold a dead value");
-    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-    const int s = (int)(blk & ((1u << lstrips) - 1));
-    const int o0 = (int)(blk >> lstrips) * 256 * K + (int)threadIdx.x;
+    const unsigned blk = order == 1 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+    const unsigned nls = gridDim.x >> lstrips;  // line sets
+    const int s = order == 2 ? (int)(blk / nls) : (int)(blk & ((1u << lstrips) - 1));
+    const int o0 = (int)(order == 2 ? blk % nls : blk >> lstrips) * 256 * K + (int)threadIdx.x;
     // the launch gives K > 1 only when 256 K divides `lines`
     if (o0 >= lines) return;
     const int64_t fb = (int64_t)blockIdx.y * g.field;
@@ -349,17 +353,18 @@ __global__ void __launch_bounds__(256) deriv4_march_kernel(const E* __restrict__
 // A K that does not divide the line count falls back to K=1.
 template <typename E, typename S, int AXIS, int K, int RING, int AHEAD, bool NTL>
 static hipError_t launch_march(const E* in, E* out, const Geo& g, int batch, int lines, int len, int tmax, S denom,
-                               bool negate, hipStream_t stream) {
+                               bool negate, hipStream_t stream, int order = 0) {
     if constexpr (K > 1) {
         if (lines % (256 * K))
-            return launch_march<E, S, AXIS, 1, 16, 12, NTL>(in, out, g, batch, lines, len, tmax, denom, negate, stream);
+            return launch_march<E, S, AXIS, 1, 16, 12, NTL>(in, out, g, batch, lines, len, tmax, denom, negate, stream,
+                                                            order);
     }
     const int T = len < tmax ? len : tmax;
     int lstrips = 0;
     while ((T << lstrips) < len) lstrips++;
     const unsigned blocks = (unsigned)(((int64_t)lines + 256 * K - 1) / (256 * K)) << lstrips;
     hipLaunchKernelGGL((deriv4_march_kernel<E, S, AXIS, K, RING, AHEAD, NTL>), dim3(blocks, (unsigned)batch),
-                       dim3(256), 0, stream, in, out, g, T, lstrips, lines, denom, negate ? 1 : 0);
+                       dim3(256), 0, stream, in, out, g, T, lstrips, lines, denom, negate ? 1 : 0, order);
     return hipGetLastError();
 }
 
@@ -369,22 +374,31 @@ static hipError_t launch_deriv4(const E* in, E* out, const Geo& g, int batch, in
     const bool d3 = g.dims == 3;
     const int planes = d3 ? g.n : 1;
     constexpr bool wide = sizeof(E) >= 16;  // complex FP64
+    // Block orders from a same-box comparison of the three at 8192^2 (P = 4,
+    // B = 32) and 512^3 (P = 2, B = 16), complex FP32 / FP64: plain order for
+    // x in 3D (0.390 -> 0.379, 0.787 -> 0.748 ms) and 2D FP64 (0.456 ->
+    // 0.432 ms) and for y in 3D (0.405 -> 0.391, 0.819 -> 0.803 ms);
+    // consecutive line sets for z in FP64 (0.826 -> 0.799 ms); the rest keep
+    // XCD-aware strips (2D FP32 x plain: 0.200 -> 0.218 ms, 2D FP64 y plain:
+    // 0.418 -> 0.499 ms)
     if (axis == 0) {
         const int lines = planes * g.n * g.vec, len = g.b * g.p;
-        if (d3) return launch_march<E, S, 0, 1, 16, 12, false>(in, out, g, batch, lines, len, 16, denom, negate, stream);
+        if (d3)
+            return launch_march<E, S, 0, 1, 16, 12, false>(in, out, g, batch, lines, len, 16, denom, negate, stream, 1);
         return launch_march<E, S, 0, 1, 16, 12, true>(in, out, g, batch, lines, len, wide ? 32 : 128, denom, negate,
-                                                      stream);
+                                                      stream, wide ? 1 : 0);
     }
     if (axis == 1) {
         const int lines = planes * g.n;
         if (d3)
             return launch_march<E, S, 1, 1, 16, 12, false>(in, out, g, batch, lines, g.n, wide ? 32 : 16, denom, negate,
-                                                           stream);
+                                                           stream, 1);
         if (wide) return launch_march<E, S, 1, 2, 16, 12, false>(in, out, g, batch, lines, g.n, 32, denom, negate, stream);
         return launch_march<E, S, 1, 2, 8, 6, true>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
     }
     const int lines = g.n * g.n;
-    if (wide) return launch_march<E, S, 2, 4, 8, 5, false>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
+    if (wide)
+        return launch_march<E, S, 2, 4, 8, 5, false>(in, out, g, batch, lines, g.n, 256, denom, negate, stream, 2);
     return launch_march<E, S, 2, 4, 8, 6, true>(in, out, g, batch, lines, g.n, 256, denom, negate, stream);
 }
 
