@@ -614,6 +614,9 @@ def main():
         "step_hbm_frac_moved": round(extra["step_chain"]["kernel_sweeps_per_step"] * sweep / (ms * 1e-3) / 1e9
                                      / HBM_PEAK_GBS, 4),
     })
+    moved = extra["step_hbm_frac_moved"]
+    extra["target_hbm_frac"] = {"target": 0.70, "measured": moved, "met": moved >= 0.70,
+                                "basis": "step_hbm_frac_moved: the bytes the step's kernels move over its time"}
     if plan.shape.b >= 2:
         extra["derivatives"] = time_derivatives(plan, x, n, 10, torch)
 
